@@ -1,0 +1,171 @@
+"""ctypes front-end of the CPU checkers (test infrastructure only).
+
+``Oracle`` wraps ``oracle/liboracle.so`` (the FP64 restatement in
+trtt_oracle.c); ``Reference`` wraps ``oracle/_ref/libtrtt_ref.so`` (the
+reference C plugin compiled from its own sources by oracle/Makefile) when that
+library has been built.  Only tests, ``__graft_entry__.smoke()`` and the CPU
+baseline of ``bench.py`` import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libtrtt_ref.so")
+
+dp = C.POINTER(C.c_double)
+L, I, D = C.c_long, C.c_int, C.c_double
+
+FP_CUBIC, FP_TABLE3D, FP_TABLE2D = 0, 1, 2
+
+
+def _p(a):
+    return a.ctypes.data_as(dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle (and the reference where its sources exist)."""
+    if force or not os.path.exists(ORACLE_SO):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class LF2DParams(C.Structure):
+    """Field order of lf2d_parameters_t (LongFessler2D.h:16-28)."""
+
+    _fields_ = [("pixelSize", D), ("pixelStep", D), ("voxelSize", D), ("sourceDistance", D),
+                ("detectorDistance", D), ("objectOffset1", D), ("objectOffset2", D),
+                ("objectDimension1", L), ("objectDimension2", L), ("detectorDimension", L)]
+
+
+class Oracle:
+    """FP64 restatement of the projectors (trtt_oracle.c)."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        lib.orc_bspline3_cdf.argtypes = [D]
+        lib.orc_bspline3_cdf.restype = D
+        lib.orc_bspline3.argtypes = [D]
+        lib.orc_bspline3.restype = D
+        lib.orc_footprint_table.argtypes = [L, I, dp, dp, dp, dp]
+        lib.orc_footprint_table.restype = None
+        lib.orc_sb3d_ex.argtypes = [dp, dp, L, L, L, D, I, I, dp, L, D, D, L, L, D, D, dp, dp, I, I]
+        lib.orc_sb3d_ex.restype = I
+        lib.orc_sb2d_ex.argtypes = [dp, dp, L, L, D, I, I, dp, L, D, D, L, D, dp, dp, I, I]
+        lib.orc_sb2d_ex.restype = I
+        lib.orc_sb3d_views_fwd.argtypes = [dp, dp, L, L, L, D, I, I, dp, L, D, D, L, L, D, D, dp, dp,
+                                           L, I]
+        lib.orc_sb3d_views_fwd.restype = I
+        lib.orc_sb3d_count_pairs.argtypes = [L, L, L, D, I, L, L, D, D, dp, dp, I]
+        lib.orc_sb3d_count_pairs.restype = D
+        lib.orc_lf2d.argtypes = [C.POINTER(LF2DParams), D, D, dp, dp, I, I]
+        lib.orc_lf2d.restype = None
+        self.lib = lib
+
+    # -- scalar helpers -------------------------------------------------------
+    def cdf(self, x: float) -> float:
+        return self.lib.orc_bspline3_cdf(float(x))
+
+    def bspline3(self, x: float) -> float:
+        return self.lib.orc_bspline3(float(x))
+
+    def footprint_table(self, m: int, deg: int = 3):
+        fp, cum, coord, step = np.zeros(m), np.zeros(m), np.zeros(m), np.zeros(1)
+        self.lib.orc_footprint_table(m, deg, _p(fp), _p(cum), _p(coord), _p(step))
+        return fp, cum, coord, float(step[0])
+
+    # -- projectors -----------------------------------------------------------
+    def sb3d(self, out, inp, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, cone, job,
+             table=None, deg=3):
+        """One view of trtt_{PB,CB}3D[_cubic]; accumulates into ``out``."""
+        kind, cum, m, step, c0 = self._fp(table, FP_TABLE3D)
+        xos, xsd, inp = _f64(xos), _f64(xsd), _f64(inp)
+        assert out.dtype == np.float64 and out.flags.c_contiguous
+        r = self.lib.orc_sb3d_ex(_p(out), _p(inp), nx, ny, nz, s, deg, kind, _p(cum), m, step, c0,
+                                 nu, nv, us, vs, _p(xos), _p(xsd), int(cone), int(job))
+        if r != 0:
+            raise RuntimeError("oracle projector failure")
+        return out
+
+    def sb2d(self, out, inp, nx, ny, s, nv, vs, xos, xsd, fan, job, table=None, deg=3):
+        """One view of trtt_{PB,FB}2D[_cubic]; accumulates into ``out``."""
+        kind, cum, m, step, c0 = self._fp(table, FP_TABLE2D)
+        xos, xsd, inp = _f64(xos), _f64(xsd), _f64(inp)
+        assert out.dtype == np.float64 and out.flags.c_contiguous
+        r = self.lib.orc_sb2d_ex(_p(out), _p(inp), nx, ny, s, deg, kind, _p(cum), m, step, c0, nv, vs,
+                                 _p(xos), _p(xsd), int(fan), int(job))
+        if r != 0:
+            raise RuntimeError("oracle projector failure")
+        return out
+
+    def views_fwd3d(self, vol, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, cone, table=None, deg=3):
+        """Forward projection of every view (OpenMP over views)."""
+        kind, cum, m, step, c0 = self._fp(table, FP_TABLE3D)
+        nview = xos.shape[0]
+        sino = np.zeros((nview, nv, nu))
+        r = self.lib.orc_sb3d_views_fwd(_p(sino), _p(_f64(vol)), nx, ny, nz, s, deg, kind, _p(cum), m,
+                                        step, c0, nu, nv, us, vs, _p(_f64(xos)), _p(_f64(xsd)),
+                                        nview, int(cone))
+        if r != 0:
+            raise RuntimeError("oracle projector failure")
+        return sino
+
+    def count_pairs(self, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, cone, deg=3) -> float:
+        return self.lib.orc_sb3d_count_pairs(nx, ny, nz, s, deg, nu, nv, us, vs, _p(_f64(xos)),
+                                             _p(_f64(xsd)), int(cone))
+
+    def lf2d(self, params: LF2DParams, beta, offset, src, dst, clr, job):
+        src = _f64(src)
+        assert dst.dtype == np.float64 and dst.flags.c_contiguous
+        self.lib.orc_lf2d(C.byref(params), float(beta), float(offset), _p(src), _p(dst), int(clr),
+                          int(job))
+        return dst
+
+    @staticmethod
+    def _fp(table, kind):
+        if table is None:
+            return FP_CUBIC, np.zeros(1), 1, 1.0, 0.0
+        cum, coord, step = table
+        cum = _f64(cum)
+        return kind, cum, cum.size, float(step), float(coord[0])
+
+
+class Reference:
+    """The reference C plugin built from its sources (oracle/_ref)."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = C.CDLL(path)
+        tab3 = [dp, dp, L, L, L, D, I, dp, dp, L, D, dp, L, L, D, D, dp, dp, I]
+        cub3 = [dp, dp, L, L, L, D, I, L, L, D, D, dp, dp, I]
+        tab2 = [dp, dp, L, L, D, I, dp, dp, L, D, dp, L, D, dp, dp, I]
+        cub2 = [dp, dp, L, L, D, I, L, D, dp, dp, I]
+        for n, a in (("trtt_PB3D", tab3), ("trtt_CB3D", tab3), ("trtt_PB3D_cubic", cub3),
+                     ("trtt_CB3D_cubic", cub3), ("trtt_PB2D", tab2), ("trtt_FB2D", tab2),
+                     ("trtt_PB2D_cubic", cub2), ("trtt_FB2D_cubic", cub2)):
+            getattr(lib, n).argtypes = a
+            getattr(lib, n).restype = I
+        self.lib = lib
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_SO)
+
+    def CB3D_cubic(self, out, inp, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, job, deg=3):
+        return self.lib.trtt_CB3D_cubic(_p(out), _p(_f64(inp)), nx, ny, nz, s, deg, nu, nv, us, vs,
+                                        _p(_f64(xos)), _p(_f64(xsd)), job)
+
+    def PB3D_cubic(self, out, inp, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, job, deg=3):
+        return self.lib.trtt_PB3D_cubic(_p(out), _p(_f64(inp)), nx, ny, nz, s, deg, nu, nv, us, vs,
+                                        _p(_f64(xos)), _p(_f64(xsd)), job)

@@ -1,0 +1,326 @@
+"""GPU parity of libtrtt_hip.so against the reference golden vectors and the
+FP64 oracle.  Parity measure: max|gpu - ref| / max|ref| per output.
+
+Tolerances (written per test):
+  FP32 plans / drop-ins   1e-5   (north-star bound for FP32 projections)
+  FP64 plans / drop-ins   1e-12  (table mode reproduces the reference's FP64
+                                  coordinates and floor() indices)
+  adjoint <Ax,y> = <x,A'y> with FP64 dot products: 1e-6 (FP32), 1e-12 (FP64)
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import CASES2D, CASES3D, rel_err
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from yoricktrtt_amd import _lib
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return _lib.load()
+
+
+def _p(a):
+    from yoricktrtt_amd import _lib
+
+    return a.ctypes.data_as(_lib.dp)
+
+
+@pytest.fixture(params=["f32", "f64"])
+def precision(request, lib):
+    from yoricktrtt_amd import _lib
+
+    lib.trtt_hip_set_precision(_lib.F64 if request.param == "f64" else _lib.F32)
+    yield request.param
+    lib.trtt_hip_set_precision(_lib.F32)
+
+
+TOL = {"f32": 1e-5, "f64": 1e-12}
+
+
+@pytest.mark.parametrize("name", CASES3D)
+@pytest.mark.parametrize("mode", ["cubic", "table"])
+def test_dropin_3d_vs_reference(lib, golden3d, precision, name, mode):
+    g = golden3d
+    _say("dropin3d", name, mode, precision)
+    vol = np.ascontiguousarray(g["vol"])
+    nz, ny, nx = vol.shape
+    nu, nv = (int(v) for v in g[f"{name}/dims"])
+    us, vs = (float(v) for v in g[f"{name}/scl"])
+    cone = name.startswith("cone")
+    pre = "CB3D" if cone else "PB3D"
+    cum, coord, step = g["cum"], g["coord"], float(g["step"])
+    fp = np.zeros_like(cum)
+    for v in range(g[f"{name}/xos"].shape[0]):
+        xos = np.ascontiguousarray(g[f"{name}/xos"][v])
+        xsd = np.ascontiguousarray(g[f"{name}/xsd"][v])
+        for job in (0, 1):
+            inp = vol if job == 0 else np.ascontiguousarray(g[f"{name}/sino_in"][v])
+            out = np.zeros((nv, nu)) if job == 0 else np.zeros((nz, ny, nx))
+            if mode == "cubic":
+                r = getattr(lib, f"trtt_{pre}_cubic")(_p(out), _p(inp), nx, ny, nz, 1.0, 3, nu, nv, us,
+                                                     vs, _p(xos), _p(xsd), job)
+            else:
+                r = getattr(lib, f"trtt_{pre}")(_p(out), _p(inp), nx, ny, nz, 1.0, 3, _p(fp), _p(cum),
+                                               cum.size, step, _p(coord), nu, nv, us, vs, _p(xos),
+                                               _p(xsd), job)
+            assert r == 0
+            ref = g[f"{name}/{mode}/{'fwd' if job == 0 else 'bwd'}"][v]
+            assert rel_err(out, ref) < TOL[precision], (v, job)
+
+
+@pytest.mark.parametrize("name", CASES2D)
+@pytest.mark.parametrize("mode", ["cubic", "table"])
+def test_dropin_2d_vs_reference(lib, golden2d, precision, name, mode):
+    g = golden2d
+    _say("dropin2d", name, mode, precision)
+    img = np.ascontiguousarray(g["img"])
+    ny, nx = img.shape
+    nv, vs = int(g[f"{name}/nv"]), float(g[f"{name}/vs"])
+    pre = "FB2D" if name.startswith("fan") else "PB2D"
+    cum, coord, step = g["cum"], g["coord"], float(g["step"])
+    fp = np.zeros_like(cum)
+    for v in range(g[f"{name}/xos"].shape[0]):
+        xos = np.ascontiguousarray(g[f"{name}/xos"][v])
+        xsd = np.ascontiguousarray(g[f"{name}/xsd"][v])
+        for job in (0, 1):
+            inp = img if job == 0 else np.ascontiguousarray(g[f"{name}/sino_in"][v])
+            out = np.zeros(nv) if job == 0 else np.zeros((ny, nx))
+            if mode == "cubic":
+                r = getattr(lib, f"trtt_{pre}_cubic")(_p(out), _p(inp), nx, ny, 1.0, 3, nv, vs,
+                                                     _p(xos), _p(xsd), job)
+            else:
+                r = getattr(lib, f"trtt_{pre}")(_p(out), _p(inp), nx, ny, 1.0, 3, _p(fp), _p(cum),
+                                               cum.size, step, _p(coord), nv, vs, _p(xos), _p(xsd),
+                                               job)
+            assert r == 0
+            ref = g[f"{name}/{mode}/{'fwd' if job == 0 else 'bwd'}"][v]
+            assert rel_err(out, ref) < TOL[precision], (v, job)
+            if mode == "cubic" and job == 0:
+                assert out[-1] == 0.0  # last bin quirk
+
+
+def test_dropin_accumulates(lib, golden3d):
+    """The drop-ins add into `out` like the reference (out += A x)."""
+    g = golden3d
+    _say("dropin accumulate")
+    vol = np.ascontiguousarray(g["vol"])
+    nz, ny, nx = vol.shape
+    nu, nv = (int(v) for v in g["cone_a/dims"])
+    xos = np.ascontiguousarray(g["cone_a/xos"][0])
+    xsd = np.ascontiguousarray(g["cone_a/xsd"][0])
+    out = np.full((nv, nu), 2.5)
+    assert lib.trtt_CB3D_cubic(_p(out), _p(vol), nx, ny, nz, 1.0, 3, nu, nv, 1.0, 1.0, _p(xos),
+                               _p(xsd), 0) == 0
+    assert rel_err(out - 2.5, g["cone_a/cubic/fwd"][0]) < 1e-5
+
+
+def test_dropin_source_plane_failure(lib):
+    """A voxel centre in the source plane returns TRTT_FAILURE
+    (trtt_3D_projectors.c:1264-1284)."""
+    from yoricktrtt_amd.geometry import cone_geometry
+
+    _say("source-plane failure")
+    nx = ny = nz = 5
+    g = cone_geometry(nx, ny, nz, 1.0, 8, 8, 1.0, 1.0, [0.0], 0.0, 10.0)  # Rsc = 0
+    vol = np.ones((nz, ny, nx))
+    out = np.zeros((8, 8))
+    r = lib.trtt_CB3D_cubic(_p(out), _p(vol), nx, ny, nz, 1.0, 3, 8, 8, 1.0, 1.0,
+                            _p(np.ascontiguousarray(g.xos[0])), _p(np.ascontiguousarray(g.xsd[0])), 0)
+    assert r == -1
+
+
+def _lf_params(n1, n2, nd, ps, pst, vs, R, D, o1=0.0, o2=0.0):
+    from yoricktrtt_amd._lib import LF2DParameters
+
+    return LF2DParameters(ps, pst, vs, R, D, o1, o2, n1, n2, nd)
+
+
+def test_dropin_lf2d_vs_oracle(lib, oracle, precision):
+    from oracle.oracle import LF2DParams
+
+    _say("lf2d dropin", precision)
+    rng = np.random.default_rng(3)
+    args = (23, 19, 71, 0.9, 1.0, 1.3, 60.0, 40.0, 0.7, -1.1)
+    p = _lf_params(*args)
+    po = LF2DParams(args[3], args[4], args[5], args[6], args[7], args[8], args[9], args[0], args[1],
+                    args[2])
+    for beta in (0.0, 0.9, 2.5, 5.1):
+        x = rng.uniform(size=(19, 23))
+        y = rng.uniform(size=71)
+        hx = np.zeros(71)
+        lib.lf2d_direct(C.byref(p), beta, 1.5, _p(x), _p(hx), 1)
+        hty = np.zeros((19, 23))
+        lib.lf2d_adjoint(C.byref(p), beta, 1.5, _p(y), _p(hty), 1)
+        rhx = oracle.lf2d(po, beta, 1.5, x, np.zeros(71), 1, 0)
+        rhty = oracle.lf2d(po, beta, 1.5, y, np.zeros((19, 23)), 1, 1)
+        assert rel_err(hx, rhx) < TOL[precision]
+        assert rel_err(hty, rhty) < TOL[precision]
+
+
+def _dev(a, dtype):
+    return torch.tensor(a, dtype=dtype, device="cuda:0")
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel3d"])
+@pytest.mark.parametrize("model", ["cubic", "table"])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_batched_plan_vs_oracle(lib, oracle, kind, model, prec):
+    """All views in one launch: equal to the oracle view by view."""
+    from yoricktrtt_amd.geometry import cone_geometry, footprint_table, parallel_geometry_3d
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("batched", kind, model, prec)
+    nx, ny, nz, nu, nv, V = 40, 36, 30, 44, 52, 12
+    th = np.arange(V) * (2 * math.pi / V) + 0.1
+    if kind == "cone":
+        geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, th, 120.0, 180.0)
+    else:
+        geo = parallel_geometry_3d(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, th)
+    rng = np.random.default_rng(11)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    P = Projector(geo, model=model, precision=prec, size_footprint=10000)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, dt)).double().cpu().numpy()
+    tab = None
+    if model == "table":
+        _, cum, coord, step = footprint_table(10000)
+        tab = (cum, coord, step)
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v],
+                         geo.xsd[v], kind == "cone", 0, tab)
+        assert rel_err(fwd[v], rf) < TOL[prec], v
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v],
+                    kind == "cone", 1, tab)
+    assert rel_err(bwd, rb) < TOL[prec]
+    P.close()
+
+
+@pytest.mark.parametrize("kind", ["fan", "parallel2d"])
+@pytest.mark.parametrize("model", ["cubic", "table"])
+def test_batched_plan_2d_vs_oracle(lib, oracle, kind, model):
+    from yoricktrtt_amd.geometry import fan_geometry_2d, footprint_table, parallel_geometry_2d
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("batched2d", kind, model)
+    nx, ny, nv, V = 70, 64, 97, 40
+    th = np.arange(V) * (2 * math.pi / V)
+    if kind == "fan":
+        geo = fan_geometry_2d(nx, ny, 1.0, nv, 1.0, th, 200.0, 300.0)
+    else:
+        geo = parallel_geometry_2d(nx, ny, 1.0, nv, 1.0, th)
+    rng = np.random.default_rng(12)
+    img = rng.uniform(size=(ny, nx))
+    sino = rng.uniform(size=(V, nv))
+    P = Projector(geo, model=model, precision="f32")
+    fwd = P.forward(_dev(img, torch.float32)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+    tab = None
+    if model == "table":
+        _, cum, coord, step = footprint_table(10000)
+        tab = (cum, coord, step)
+    rb = np.zeros((ny, nx))
+    for v in range(V):
+        rf = oracle.sb2d(np.zeros(nv), img, nx, ny, 1.0, nv, 1.0, geo.xos[v], geo.xsd[v],
+                         kind == "fan", 0, tab)
+        assert rel_err(fwd[v], rf) < 1e-5, v
+        oracle.sb2d(rb, sino[v], nx, ny, 1.0, nv, 1.0, geo.xos[v], geo.xsd[v], kind == "fan", 1, tab)
+    assert rel_err(bwd, rb) < 1e-5
+    P.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_adjoint_identity(lib, prec):
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("adjoint", prec)
+    nx, ny, nz, nu, nv, V = 64, 64, 48, 80, 96, 30
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, np.arange(V) * (2 * math.pi / V),
+                        200.0, 300.0)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = torch.Generator(device="cuda:0").manual_seed(5)
+    x = torch.rand((nz, ny, nx), generator=g, device="cuda:0", dtype=dt)
+    y = torch.rand((V, nv, nu), generator=g, device="cuda:0", dtype=dt)
+    P = Projector(geo, model="cubic", precision=prec)
+    Ax = P.forward(x)
+    Aty = P.backward(y)
+    lhs = torch.sum(Ax.double() * y.double()).item()
+    rhs = torch.sum(x.double() * Aty.double()).item()
+    tol = 1e-6 if prec == "f32" else 1e-12
+    assert abs(lhs - rhs) <= tol * abs(lhs), (lhs, rhs)
+    P.close()
+
+
+def test_time_interpolated_phases(lib, oracle):
+    """4-D series: view k sees w0 x[t0] + w1 x[t1]; back-projection splits
+    with the same weights (one-hot and fractional weights)."""
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("phases")
+    nx, ny, nz, nu, nv, nt, V = 24, 22, 20, 30, 34, 4, 14
+    rng = np.random.default_rng(21)
+    th = rng.uniform(0, 2 * math.pi, size=V)
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, th, 90.0, 140.0)
+    t = rng.uniform(0, nt - 1, size=V)
+    t[:5] = np.floor(t[:5])  # integer (phase-binned) views
+    t0 = np.floor(t).astype(np.int32)
+    t1 = np.minimum(t0 + 1, nt - 1).astype(np.int32)
+    w1 = t - t0
+    w1[t0 == t1] = 0.0
+    w0 = 1.0 - w1
+    x = rng.uniform(size=(nt, nz, ny, nx))
+    y = rng.uniform(size=(V, nv, nu))
+    P = Projector(geo, model="cubic", precision="f64", nt=nt, t0=t0, t1=t1, w0=w0, w1=w1)
+    fwd = P.forward(_dev(x, torch.float64)).cpu().numpy()
+    bwd = P.backward(_dev(y, torch.float64)).cpu().numpy()
+    rb = np.zeros((nt, nz, ny, nx))
+    for v in range(V):
+        xv = w0[v] * x[t0[v]] + w1[v] * x[t1[v]]
+        rf = oracle.sb3d(np.zeros((nv, nu)), xv, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v],
+                         geo.xsd[v], True, 0)
+        assert rel_err(fwd[v], rf) < 1e-12
+        bv = oracle.sb3d(np.zeros((nz, ny, nx)), y[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v],
+                         geo.xsd[v], True, 1)
+        rb[t0[v]] += w0[v] * bv
+        rb[t1[v]] += w1[v] * bv
+    assert rel_err(bwd, rb) < 1e-12
+    P.close()
+
+
+def test_weighted_residual(lib):
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("residual")
+    geo = cone_geometry(8, 8, 8, 1.0, 10, 12, 1.0, 1.0, [0.0, 1.0, 2.0], 50.0, 80.0)
+    P = Projector(geo, precision="f32")
+    g = torch.Generator(device="cuda:0").manual_seed(2)
+    s = torch.rand(P.sino_shape, generator=g, device="cuda:0")
+    y = torch.rand(P.sino_shape, generator=g, device="cuda:0")
+    w = torch.rand(P.sino_shape, generator=g, device="cuda:0")
+    fx = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    d = (s - y).double()
+    ref_fx = 0.5 * torch.sum(w.double() * d * d).item()
+    ref_r = (w.double() * d)
+    P.weighted_residual(s, y, w, fx)
+    assert abs(fx.item() - ref_fx) < 1e-6 * ref_fx
+    assert rel_err(s.double().cpu().numpy(), ref_r.cpu().numpy()) < 1e-6
+    P.close()

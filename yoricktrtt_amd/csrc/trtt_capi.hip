@@ -1,0 +1,684 @@
+// C ABI of libtrtt_hip.so: plans (device-resident batched projectors) and the
+// reference plugin symbols (synchronous per-view drop-ins on host FP64 data).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "trtt_core.h"
+
+namespace trtt {
+hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const DevPlan &P, int job,
+                            const void *in, void *out, const int *seg_begin, int nrows,
+                            int views_per_block, int atomic_out, hipStream_t st);
+hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long n, double *fx,
+                           hipStream_t st);
+}  // namespace trtt
+
+using trtt::ViewConst;
+
+namespace {
+
+thread_local std::string g_last_error;
+int g_device = 0;
+int g_precision = TRTT_HIP_F32;
+std::mutex g_mutex;
+
+int fail(const std::string &msg) {
+  g_last_error = msg;
+  std::fprintf(stderr, "trtt_hip: %s\n", msg.c_str());
+  return TRTT_FAILURE;
+}
+
+bool hip_ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  fail(std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Per-view constants (FP64, reference expression order)
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+void view_consts_3d(ViewConst &g, const double *xos, const double *xsd, bool cone, double s,
+                    int s_deg, double u_scl, double v_scl) {
+  // trtt_3D_projectors.c:1190-1232 (cone), :967-1003 (parallel)
+  const double a1 = xsd[0], a2 = xsd[1], a3 = xsd[2], a4 = xsd[3];
+  const double a5 = xsd[4], a6 = xsd[5], a7 = xsd[6], a8 = xsd[7];
+  const double a9 = xsd[8], a10 = xsd[9], a11 = xsd[10], a12 = xsd[11];
+  const double a1inv = -1.0 * (1.0 / a1);
+  const double b1 = a6 + a1inv * a5 * a2, c1 = a10 + a1inv * a9 * a2;
+  const double b2 = a7 + a1inv * a5 * a3, c2 = a11 + a1inv * a9 * a3;
+  const double b3 = a8 + a1inv * a5 * a4, c3 = a12 + a1inv * a9 * a4;
+  std::memset(&g, 0, sizeof(g));
+  g.nvx = b1 * xos[4] + b2 * xos[8];
+  g.nux = c1 * xos[4] + c2 * xos[8];
+  g.nvy = b1 * xos[5] + b2 * xos[9];
+  g.nuy = c1 * xos[5] + c2 * xos[9];
+  g.nvz = b1 * xos[6] + b2 * xos[10];
+  g.nuz = c1 * xos[6] + c2 * xos[10];
+  if (cone) {
+    g.nvc = b1 * xos[7] + b2 * xos[11];
+    g.nuc = c1 * xos[7] + c2 * xos[11];
+  } else {
+    g.nvc = b1 * xos[7] + b2 * xos[11] + b3;
+    g.nuc = c1 * xos[7] + c2 * xos[11] + c3;
+  }
+  g.dx = a1 * xos[0] + a2 * xos[4] + a3 * xos[8];
+  g.dy = a1 * xos[1] + a2 * xos[5] + a3 * xos[9];
+  g.dz = a1 * xos[2] + a2 * xos[6] + a3 * xos[10];
+  g.dc = a1 * xos[3] + a2 * xos[7] + a3 * xos[11];
+  g.o1 = xos[0];
+  g.o2 = xos[1];
+  g.o3 = xos[2];
+  g.o4 = xos[3];
+  g.a4m = -1.0 * a4;
+  g.a4sqr = a4 * a4;
+  g.a8 = a8;
+  g.a12 = a12;
+  g.b3 = b3;
+  g.c3 = c3;
+  const double hs0 = 0.5 * (s_deg + 1);
+  const double a1sqr = a1 * a1;
+  const double a5sqr = v_scl * v_scl * a5 * a5;
+  const double a9sqr = u_scl * u_scl * a9 * a9;
+  const double du = std::sqrt(1.0 + (a9sqr / (a1sqr + a5sqr)));
+  const double dv = std::sqrt(1.0 + (a5sqr / a1sqr));
+  g.pb_rho0 = s * s * s * du * dv;
+  g.pb_hu = s / u_scl;
+  g.pb_hv = s / v_scl;
+  g.pb_su = 1.0 / g.pb_hu;
+  g.pb_sv = 1.0 / g.pb_hv;
+  g.pb_hsu = hs0 * g.pb_hu;
+  g.pb_hsv = hs0 * g.pb_hv;
+}
+
+void view_consts_2d(ViewConst &g, const double *xos, const double *xsd, bool fan, bool table,
+                    double step, double s, int s_deg, double v_scl) {
+  // trtt_2D_projectors.c:195-221 (parallel), :279-315 (fan)
+  const double a1 = xsd[0], a2 = xsd[1], a4 = xsd[3], a5 = xsd[4], a6 = xsd[5], a8 = xsd[7];
+  const double a1inv = -1.0 * (1.0 / a1);
+  std::memset(&g, 0, sizeof(g));
+  if (fan) {
+    const double b1 = a6 + a1inv * a5 * a2;
+    const double b3 = a8 + a1inv * a5 * a4;
+    g.nvx = b1 * xos[4];
+    g.nvy = b1 * xos[5];
+    g.nvc = b1 * xos[7];
+    g.dx = a1 * xos[0] + a2 * xos[4];
+    g.dy = a1 * xos[1] + a2 * xos[5];
+    g.dc = a1 * xos[3] + a2 * xos[7];
+    g.o1 = xos[0];
+    g.o2 = xos[1];
+    g.o4 = xos[3];
+    g.a4m = -1.0 * a4;
+    g.a4sqr = a4 * a4;
+    g.a8 = a8;
+    g.b3 = b3;
+  } else {
+    const double a5a1 = a5 * a1inv;
+    const double a5a1scl = a5a1 * v_scl;
+    const double b1 = a6 + a5a1 * a2;
+    const double b3 = a8 + a5a1 * a4;
+    g.nvx = b1 * xos[4];
+    g.nvy = b1 * xos[5];
+    g.nvc = b1 * xos[7] + b3;
+    const double dv = std::sqrt(1 + a5a1scl * a5a1scl);
+    g.pb_rho0 = s * s * dv;
+    g.pb_hv = (dv * s) / v_scl;
+    g.pb_sv = table ? 1 / (g.pb_hv * step) : 1 / g.pb_hv;
+    g.pb_hsv = 0.5 * (s_deg + 1) * g.pb_hv;
+  }
+}
+
+void view_consts_lf(ViewConst &g, const lf2d_parameters_t *p, double beta, double det_offset) {
+  // LongFessler2D.c:59-75
+  std::memset(&g, 0, sizeof(g));
+  const double cb = std::cos(beta), sb = std::sin(beta);
+  const double oy = cb * p->objectOffset1 + sb * p->objectOffset2;
+  const double ox = -sb * p->objectOffset1 + cb * p->objectOffset2;
+  g.lf_cb = cb;
+  g.lf_sb = sb;
+  g.lf_x0 = ox - 0.5 * (p->objectDimension1 - 1) * p->voxelSize;
+  g.lf_y0 = oy - 0.5 * (p->objectDimension2 - 1) * p->voxelSize;
+  g.lf_off = det_offset / p->pixelStep + 0.5 * (p->detectorDimension - 1);
+  g.w0 = 1.0;
+}
+
+#pragma clang fp contract(on)
+
+// Number of workgroups a launch should offer to fill 256 CUs several times.
+constexpr long kTargetBlocks = 4096;
+
+int tiles_of(const trtt_hip_plan *p) {
+  const bool is2d = p->kind == trtt::PB2D || p->kind == trtt::FB2D || p->kind == trtt::LF2D;
+  const long tx = is2d ? 16 : 8, ty = is2d ? 16 : 8, tz = is2d ? 1 : 8;
+  return int(((p->nx + tx - 1) / tx) * ((p->ny + ty - 1) / ty) * ((p->nz + tz - 1) / tz));
+}
+
+// Group views by (t0, t1) and cut the ordered list into `nrows` back-
+// projection rows of contiguous segments.
+void build_segments(trtt_hip_plan *p) {
+  const long V = p->nviews;
+  p->h_order.resize(V);
+  std::iota(p->h_order.begin(), p->h_order.end(), 0);
+  std::stable_sort(p->h_order.begin(), p->h_order.end(), [&](int a, int b) {
+    const auto &A = p->h_views[a], &B = p->h_views[b];
+    return A.t0 != B.t0 ? A.t0 < B.t0 : A.t1 < B.t1;
+  });
+  const long ntiles = tiles_of(p);
+  long nrows = (kTargetBlocks + ntiles - 1) / ntiles;
+  nrows = std::max(1L, std::min(nrows, V));
+  if (p->kind == trtt::LF2D) nrows = 1;
+  p->h_segs.clear();
+  p->h_seg_begin.assign(nrows + 1, 0);
+  for (long r = 0; r < nrows; ++r) {
+    const long a = r * V / nrows, b = (r + 1) * V / nrows;
+    p->h_seg_begin[r] = int(p->h_segs.size());
+    long s = a;
+    while (s < b) {
+      const auto &A = p->h_views[p->h_order[s]];
+      long e = s + 1;
+      while (e < b && p->h_views[p->h_order[e]].t0 == A.t0 &&
+             p->h_views[p->h_order[e]].t1 == A.t1)
+        ++e;
+      p->h_segs.push_back(trtt::Segment{int(s), int(e - s), A.t0, A.t1});
+      s = e;
+    }
+  }
+  p->h_seg_begin[nrows] = int(p->h_segs.size());
+}
+
+template <typename T>
+bool upload(T *&dst, const std::vector<T> &src) {
+  if (src.empty()) return true;
+  if (!hip_ok(hipMalloc(&dst, sizeof(T) * src.size()), "hipMalloc")) return false;
+  return hip_ok(hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice),
+                "hipMemcpy");
+}
+
+bool finalize_plan(trtt_hip_plan *p, const double *cum) {
+  if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return false;
+  build_segments(p);
+  if (!upload(p->d_views, p->h_views) || !upload(p->d_order, p->h_order) ||
+      !upload(p->d_segs, p->h_segs) || !upload(p->d_seg_begin, p->h_seg_begin))
+    return false;
+  if (p->table) {
+    if (p->precision == TRTT_HIP_F64) {
+      std::vector<double> t(cum, cum + p->m);
+      double *d = nullptr;
+      if (!upload(d, t)) return false;
+      p->d_table = d;
+    } else {
+      std::vector<float> t(p->m);
+      for (long i = 0; i < p->m; ++i) t[i] = float(cum[i]);
+      float *d = nullptr;
+      if (!upload(d, t)) return false;
+      p->d_table = d;
+    }
+  }
+  if (!hip_ok(hipMalloc(&p->d_err, sizeof(int)), "hipMalloc")) return false;
+  if (!hip_ok(hipMemset(p->d_err, 0, sizeof(int)), "hipMemset")) return false;
+  trtt::DevPlan &P = p->dev;
+  P.views = p->d_views;
+  P.order = p->d_order;
+  P.segs = p->d_segs;
+  P.table = p->d_table;
+  P.err = p->d_err;
+  P.m = p->m;
+  P.step = p->step;
+  P.coord0 = p->coord0;
+  P.nx = int(p->nx);
+  P.ny = int(p->ny);
+  P.nz = int(p->nz);
+  P.nu = int(p->nu);
+  P.nv = int(p->nv);
+  P.nviews = int(p->nviews);
+  P.nt = int(p->nt);
+  P.vox_per_phase = p->nx * p->ny * p->nz;
+  P.u_scl = p->u_scl;
+  P.v_scl = p->v_scl;
+  const double s = p->s_scl;
+  const bool is3d = p->kind == trtt::PB3D || p->kind == trtt::CB3D;
+  {
+#pragma clang fp contract(off)
+    P.rho_init = is3d ? s * s * s : s * s;
+    P.hu_init = s / p->u_scl;
+    P.hv_init = s / p->v_scl;
+    P.hs0 = 0.5 * (p->s_deg + 1);
+  }
+  return true;
+}
+
+void release(trtt_hip_plan *p) {
+  if (!p) return;
+  hipSetDevice(p->device);
+  hipFree(p->d_views);
+  hipFree(p->d_order);
+  hipFree(p->d_segs);
+  hipFree(p->d_seg_begin);
+  hipFree(p->d_table);
+  hipFree(p->d_err);
+  delete p;
+}
+
+int views_per_block(const trtt_hip_plan *p) {
+  const long ntiles = tiles_of(p);
+  long vpb = (ntiles * p->nviews) / kTargetBlocks;
+  return int(std::max(1L, std::min(vpb, 8L)));
+}
+
+// ---------------------------------------------------------------------------
+// Scratch buffers of the synchronous drop-in symbols (grow-only, per process)
+// ---------------------------------------------------------------------------
+struct Scratch {
+  void *buf[3] = {nullptr, nullptr, nullptr};
+  size_t cap[3] = {0, 0, 0};
+  int device = -1;
+  bool ensure(int slot, size_t bytes) {
+    if (cap[slot] >= bytes) return true;
+    hipFree(buf[slot]);
+    buf[slot] = nullptr;
+    cap[slot] = 0;
+    if (!hip_ok(hipMalloc(&buf[slot], bytes), "hipMalloc")) return false;
+    cap[slot] = bytes;
+    return true;
+  }
+};
+Scratch g_scratch;
+
+__global__ void cvt_d2f(const double *__restrict__ a, float *__restrict__ b, long n) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+__global__ void cvt_f2d(const float *__restrict__ a, double *__restrict__ b, long n) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    b[i] = double(a[i]);
+}
+
+unsigned grid_for(long n) {
+  long b = (n + 255) / 256;
+  return unsigned(std::max(1L, std::min(b, 8192L)));
+}
+
+// Run one view of a plan on host FP64 arrays: out_host += A in_host (job 0)
+// or A' in_host (job != 0).
+int run_dropin(trtt_hip_plan *p, double *out, const double *in, int job) {
+  const long nvol = p->nx * p->ny * p->nz;
+  const long nsino = p->nu * p->nv;
+  const long nin = job == 0 ? nvol : nsino, nout = job == 0 ? nsino : nvol;
+  const bool f64 = p->precision == TRTT_HIP_F64;
+  const size_t esz = f64 ? 8 : 4;
+  if (g_scratch.device != p->device) {
+    for (int s = 0; s < 3; ++s) {
+      g_scratch.buf[s] = nullptr;
+      g_scratch.cap[s] = 0;
+    }
+    g_scratch.device = p->device;
+  }
+  if (!g_scratch.ensure(0, sizeof(double) * std::max(nin, nout)) ||
+      !g_scratch.ensure(1, esz * nin) || !g_scratch.ensure(2, esz * nout))
+    return TRTT_FAILURE;
+  double *d_stage = static_cast<double *>(g_scratch.buf[0]);
+  void *d_in = g_scratch.buf[1], *d_out = g_scratch.buf[2];
+  hipStream_t st = nullptr;
+  if (f64) {
+    if (!hip_ok(hipMemcpy(d_in, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
+      return TRTT_FAILURE;
+  } else {
+    if (!hip_ok(hipMemcpy(d_stage, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
+      return TRTT_FAILURE;
+    hipLaunchKernelGGL(cvt_d2f, dim3(grid_for(nin)), dim3(256), 0, st, d_stage,
+                       static_cast<float *>(d_in), nin);
+  }
+  if (!hip_ok(hipMemsetAsync(d_out, 0, esz * nout, st), "hipMemset")) return TRTT_FAILURE;
+  if (!hip_ok(hipMemsetAsync(p->d_err, 0, sizeof(int), st), "hipMemset")) return TRTT_FAILURE;
+  const int nrows = int(p->h_seg_begin.size()) - 1;
+  if (!hip_ok(trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, job, d_in,
+                                     d_out, p->d_seg_begin, nrows, views_per_block(p),
+                                     nrows > 1 ? 1 : 0, st),
+              "kernel launch"))
+    return TRTT_FAILURE;
+  std::vector<double> res(nout);
+  if (f64) {
+    if (!hip_ok(hipMemcpy(res.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost),
+                "hipMemcpy"))
+      return TRTT_FAILURE;
+  } else {
+    hipLaunchKernelGGL(cvt_f2d, dim3(grid_for(nout)), dim3(256), 0, st,
+                       static_cast<const float *>(d_out), d_stage, nout);
+    if (!hip_ok(hipMemcpy(res.data(), d_stage, sizeof(double) * nout, hipMemcpyDeviceToHost),
+                "hipMemcpy"))
+      return TRTT_FAILURE;
+  }
+  int err = 0;
+  if (!hip_ok(hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy"))
+    return TRTT_FAILURE;
+  if (err) return fail("IT WAS SUPPOSED TO BE HIGHLY IMPOSSIBLE: a voxel lies in the source plane");
+  for (long i = 0; i < nout; ++i) out[i] += res[i];
+  return TRTT_SUCCESS;
+}
+
+bool bad_sizes(long a, long b, long c, long d, long e) {
+  return a <= 0 || b <= 0 || c <= 0 || d <= 0 || e <= 0 || a > 1L << 30 || b > 1L << 30 ||
+         c > 1L << 30 || d > 1L << 30 || e > 1L << 30;
+}
+
+trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, long nt, double s_scl,
+                         int s_deg, long nu, long nv, double u_scl, double v_scl,
+                         const double *xos, const double *xsd, long nviews, const double *cum,
+                         long m, double step, double coord0, const int *t0, const int *t1,
+                         const double *w0, const double *w1) {
+  if (kind < 0 || kind > 3) {
+    fail("unknown projector kind");
+    return nullptr;
+  }
+  const bool is2d = kind == trtt::PB2D || kind == trtt::FB2D;
+  if (is2d) {
+    nz = 1;
+    nu = 1;
+  }
+  if (bad_sizes(nx, ny, nz, nu, nv) || nviews <= 0 || nt <= 0 || !xos || !xsd ||
+      (cum && m < 2)) {
+    fail("invalid plan arguments");
+    return nullptr;
+  }
+  if (nx * ny * nz * nt > (1L << 40) || nu * nv * nviews > (1L << 40)) {
+    fail("plan too large");
+    return nullptr;
+  }
+  auto *p = new trtt_hip_plan();
+  p->kind = kind;
+  p->precision = precision == TRTT_HIP_F64 ? TRTT_HIP_F64 : TRTT_HIP_F32;
+  p->device = g_device;
+  p->table = cum != nullptr;
+  p->quirk_last_bin = is2d && !p->table;
+  p->nx = nx;
+  p->ny = ny;
+  p->nz = nz;
+  p->nt = nt;
+  p->nu = nu;
+  p->nv = nv;
+  p->nviews = nviews;
+  p->s_scl = s_scl;
+  p->s_deg = s_deg;
+  p->u_scl = u_scl;
+  p->v_scl = v_scl;
+  p->m = cum ? m : 0;
+  p->step = cum ? step : 1.0;
+  p->coord0 = cum ? coord0 : 0.0;
+  p->h_views.resize(nviews);
+  for (long k = 0; k < nviews; ++k) {
+    ViewConst &g = p->h_views[k];
+    if (is2d)
+      view_consts_2d(g, xos + 12 * k, xsd + 12 * k, kind == trtt::FB2D, p->table, p->step, s_scl,
+                     s_deg, v_scl);
+    else
+      view_consts_3d(g, xos + 12 * k, xsd + 12 * k, kind == trtt::CB3D, s_scl, s_deg, u_scl,
+                     v_scl);
+    g.t0 = t0 ? t0[k] : 0;
+    g.t1 = t1 ? t1[k] : g.t0;
+    g.w0 = w0 ? w0[k] : 1.0;
+    g.w1 = w1 ? w1[k] : 0.0;
+    if (g.t0 < 0 || g.t0 >= nt || g.t1 < 0 || g.t1 >= nt) {
+      delete p;
+      fail("time index out of range");
+      return nullptr;
+    }
+  }
+  if (!finalize_plan(p, cum)) {
+    release(p);
+    return nullptr;
+  }
+  return p;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Part 2 -- plans
+// ===========================================================================
+extern "C" {
+
+int trtt_hip_set_device(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    return fail("invalid device");
+  g_device = device;
+  return TRTT_SUCCESS;
+}
+
+int trtt_hip_set_precision(int precision) {
+  if (precision != TRTT_HIP_F32 && precision != TRTT_HIP_F64) return fail("invalid precision");
+  g_precision = precision;
+  return TRTT_SUCCESS;
+}
+
+const char *trtt_hip_last_error(void) { return g_last_error.c_str(); }
+
+trtt_hip_plan_t *trtt_hip_plan_create(int kind, int precision, long nx, long ny, long nz, long nt,
+                                      double s_scl, int s_deg, long nu, long nv, double u_scl,
+                                      double v_scl, const double *xos, const double *xsd,
+                                      long nviews, const double *cum, long size_footprint,
+                                      double step_footprint, double coord0, const int *t0,
+                                      const int *t1, const double *w0, const double *w1) {
+  g_last_error.clear();
+  return make_plan(kind, precision, nx, ny, nz, nt, s_scl, s_deg, nu, nv, u_scl, v_scl, xos, xsd,
+                   nviews, cum, size_footprint, step_footprint, coord0, t0, t1, w0, w1);
+}
+
+trtt_hip_plan_t *trtt_hip_plan_create_lf2d(int precision, const lf2d_parameters_t *lp,
+                                           const double *beta, const double *offset,
+                                           long nviews) {
+  g_last_error.clear();
+  if (!lp || !beta || nviews <= 0 ||
+      bad_sizes(lp->objectDimension1, lp->objectDimension2, 1, 1, lp->detectorDimension)) {
+    fail("invalid Long & Fessler plan arguments");
+    return nullptr;
+  }
+  auto *p = new trtt_hip_plan();
+  p->kind = trtt::LF2D;
+  p->precision = precision == TRTT_HIP_F64 ? TRTT_HIP_F64 : TRTT_HIP_F32;
+  p->device = g_device;
+  p->nx = lp->objectDimension1;
+  p->ny = lp->objectDimension2;
+  p->nz = 1;
+  p->nu = 1;
+  p->nv = lp->detectorDimension;
+  p->nviews = nviews;
+  p->h_views.resize(nviews);
+  for (long k = 0; k < nviews; ++k)
+    view_consts_lf(p->h_views[k], lp, beta[k], offset ? offset[k] : 0.0);
+  if (!finalize_plan(p, nullptr)) {
+    release(p);
+    return nullptr;
+  }
+  trtt::DevPlan &P = p->dev;
+  {
+#pragma clang fp contract(off)
+    P.lf_vs = lp->voxelSize;
+    P.lf_ps = lp->pixelSize;
+    P.lf_R = lp->sourceDistance;
+    P.lf_scale = (lp->sourceDistance + lp->detectorDistance) / lp->pixelStep;
+    P.lf_size = lp->pixelSize / lp->pixelStep;
+  }
+  return p;
+}
+
+int trtt_hip_forward(trtt_hip_plan_t *p, const void *vol, void *sino, void *stream) {
+  if (!p || !vol || !sino) return fail("null argument");
+  const int nrows = int(p->h_seg_begin.size()) - 1;
+  return hip_ok(trtt::launch_projector(p->kind, p->table, p->quirk_last_bin,
+                                       p->precision == TRTT_HIP_F64, p->dev, 0, vol, sino,
+                                       p->d_seg_begin, nrows, views_per_block(p), 0,
+                                       static_cast<hipStream_t>(stream)),
+                "forward launch")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_backward(trtt_hip_plan_t *p, const void *sino, void *vol, void *stream) {
+  if (!p || !vol || !sino) return fail("null argument");
+  const int nrows = int(p->h_seg_begin.size()) - 1;
+  int vpb = int(p->nviews);
+  if (p->kind == trtt::LF2D) vpb = int(p->nviews);
+  return hip_ok(trtt::launch_projector(p->kind, p->table, p->quirk_last_bin,
+                                       p->precision == TRTT_HIP_F64, p->dev, 1, sino, vol,
+                                       p->d_seg_begin, nrows, vpb, nrows > 1 ? 1 : 0,
+                                       static_cast<hipStream_t>(stream)),
+                "backward launch")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_weighted_residual(trtt_hip_plan_t *p, void *sino, const void *y, const void *w,
+                               double *fx_dev, void *stream) {
+  if (!p || !sino || !y || !fx_dev) return fail("null argument");
+  return hip_ok(trtt::launch_residual(p->precision == TRTT_HIP_F64, sino, y, w,
+                                      p->nu * p->nv * p->nviews, fx_dev,
+                                      static_cast<hipStream_t>(stream)),
+                "residual launch")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_plan_error_flag(trtt_hip_plan_t *p) {
+  if (!p) return -1;
+  int err = 0;
+  if (hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return err;
+}
+
+long trtt_hip_plan_nviews(const trtt_hip_plan_t *p) { return p ? p->nviews : 0; }
+long trtt_hip_plan_volume_size(const trtt_hip_plan_t *p) {
+  return p ? p->nt * p->nx * p->ny * p->nz : 0;
+}
+long trtt_hip_plan_sino_size(const trtt_hip_plan_t *p) {
+  return p ? p->nviews * p->nu * p->nv : 0;
+}
+int trtt_hip_plan_destroy(trtt_hip_plan_t *p) {
+  release(p);
+  return TRTT_SUCCESS;
+}
+
+// ===========================================================================
+// Part 1 -- reference plugin symbols
+// ===========================================================================
+static int dropin(int kind, double *out, double *in, long nx, long ny, long nz, double s_scl,
+                  int s_deg, const double *cum, long m, double step, const double *coord,
+                  long nu, long nv, double u_scl, double v_scl, const double *xos,
+                  const double *xsd, int job) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!out || !in || !xos || !xsd) return fail("null argument");
+  if (cum && !coord) return fail("null footprint coordinates");
+  if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return TRTT_FAILURE;
+  trtt_hip_plan *p = make_plan(kind, g_precision, nx, ny, nz, 1, s_scl, s_deg, nu, nv, u_scl,
+                               v_scl, xos, xsd, 1, cum, m, step, cum ? coord[0] : 0.0, nullptr,
+                               nullptr, nullptr, nullptr);
+  if (!p) return TRTT_FAILURE;
+  const int r = run_dropin(p, out, in, job);
+  release(p);
+  return r;
+}
+
+int trtt_PB2D(double *out, double *in, long nx, long ny, double s_scl, int s_deg,
+              double *footprint, double *cum_footprint, long size_footprint,
+              double step_footprint, double *coord_footprint, long nv, double v_scl, double *xos,
+              double *xsd, int job) {
+  (void)footprint;
+  if (!cum_footprint) return fail("null footprint table");
+  return dropin(trtt::PB2D, out, in, nx, ny, 1, s_scl, s_deg, cum_footprint, size_footprint,
+                step_footprint, coord_footprint, 1, nv, 1.0, v_scl, xos, xsd, job);
+}
+
+int trtt_FB2D(double *out, double *in, long nx, long ny, double s_scl, int s_deg,
+              double *footprint, double *cum_footprint, long size_footprint,
+              double step_footprint, double *coord_footprint, long nv, double v_scl, double *xos,
+              double *xsd, int job) {
+  (void)footprint;
+  if (!cum_footprint) return fail("null footprint table");
+  return dropin(trtt::FB2D, out, in, nx, ny, 1, s_scl, s_deg, cum_footprint, size_footprint,
+                step_footprint, coord_footprint, 1, nv, 1.0, v_scl, xos, xsd, job);
+}
+
+int trtt_PB2D_cubic(double *out, double *in, long nx, long ny, double s_scl, int s_deg, long nv,
+                    double v_scl, double *xos, double *xsd, int job) {
+  return dropin(trtt::PB2D, out, in, nx, ny, 1, s_scl, s_deg, nullptr, 0, 1.0, nullptr, 1, nv,
+                1.0, v_scl, xos, xsd, job);
+}
+
+int trtt_FB2D_cubic(double *out, double *in, long nx, long ny, double s_scl, int s_deg, long nv,
+                    double v_scl, double *xos, double *xsd, int job) {
+  return dropin(trtt::FB2D, out, in, nx, ny, 1, s_scl, s_deg, nullptr, 0, 1.0, nullptr, 1, nv,
+                1.0, v_scl, xos, xsd, job);
+}
+
+int trtt_PB3D(double *out, double *in, long nx, long ny, long nz, double s_scl, int s_deg,
+              double *footprint, double *cum_footprint, long size_footprint,
+              double step_footprint, double *coord_footprint, long nu, long nv, double u_scl,
+              double v_scl, double *xos, double *xsd, int job) {
+  (void)footprint;
+  if (!cum_footprint) return fail("null footprint table");
+  return dropin(trtt::PB3D, out, in, nx, ny, nz, s_scl, s_deg, cum_footprint, size_footprint,
+                step_footprint, coord_footprint, nu, nv, u_scl, v_scl, xos, xsd, job);
+}
+
+int trtt_CB3D(double *out, double *in, long nx, long ny, long nz, double s_scl, int s_deg,
+              double *footprint, double *cum_footprint, long size_footprint,
+              double step_footprint, double *coord_footprint, long nu, long nv, double u_scl,
+              double v_scl, double *xos, double *xsd, int job) {
+  (void)footprint;
+  if (!cum_footprint) return fail("null footprint table");
+  return dropin(trtt::CB3D, out, in, nx, ny, nz, s_scl, s_deg, cum_footprint, size_footprint,
+                step_footprint, coord_footprint, nu, nv, u_scl, v_scl, xos, xsd, job);
+}
+
+int trtt_PB3D_cubic(double *out, double *in, long nx, long ny, long nz, double s_scl, int s_deg,
+                    long nu, long nv, double u_scl, double v_scl, double *xos, double *xsd,
+                    int job) {
+  return dropin(trtt::PB3D, out, in, nx, ny, nz, s_scl, s_deg, nullptr, 0, 1.0, nullptr, nu, nv,
+                u_scl, v_scl, xos, xsd, job);
+}
+
+int trtt_CB3D_cubic(double *out, double *in, long nx, long ny, long nz, double s_scl, int s_deg,
+                    long nu, long nv, double u_scl, double v_scl, double *xos, double *xsd,
+                    int job) {
+  return dropin(trtt::CB3D, out, in, nx, ny, nz, s_scl, s_deg, nullptr, 0, 1.0, nullptr, nu, nv,
+                u_scl, v_scl, xos, xsd, job);
+}
+
+static void lf_dropin(const lf2d_parameters_t *lp, double beta, double offset, const double *src,
+                      double *dst, int clr, int job) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!lp || !src || !dst) {
+    fail("null argument");
+    return;
+  }
+  const long nimg = lp->objectDimension1 * lp->objectDimension2;
+  const long nout = job == 0 ? lp->detectorDimension : nimg;
+  if (clr) std::memset(dst, 0, sizeof(double) * nout);
+  if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return;
+  trtt_hip_plan *p = trtt_hip_plan_create_lf2d(g_precision, lp, &beta, &offset, 1);
+  if (!p) return;
+  run_dropin(p, dst, src, job);
+  release(p);
+}
+
+void lf2d_direct(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                 const double voxel[], double pixel[], int clr) {
+  lf_dropin(p, beta, detectorOffset, voxel, pixel, clr, 0);
+}
+
+void lf2d_adjoint(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                  const double pixel[], double voxel[], int clr) {
+  lf_dropin(p, beta, detectorOffset, pixel, voxel, clr, 1);
+}
+
+}  // extern "C"

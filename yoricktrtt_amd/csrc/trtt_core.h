@@ -1,0 +1,89 @@
+// Shared definitions of the gfx950 footprint projectors: per-view constants,
+// the kernel-side plan and the host-side plan object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/trtt_hip.h"
+
+namespace trtt {
+
+enum Kind : int { PB3D = 0, CB3D = 1, PB2D = 2, FB2D = 3, LF2D = 4 };
+
+// Per-view constants, FP64, derived on the host from XOS/XSD with the
+// reference expressions (trtt_3D_projectors.c:1190-1232 for cone,
+// :967-1003 for parallel, trtt_2D_projectors.c:195-221 / :279-315 for 2-D).
+// The device evaluates, per voxel (i,j,k), in the reference's incremental
+// order ((z*k + c) + y*j) + x*i:
+//   cone/fan:  depth w = o.(i,j,k,1), numerators nu/nv, denominator d
+//   parallel:  affine uc/vc and view-constant footprint scales.
+struct ViewConst {
+  double nux, nuy, nuz, nuc;
+  double nvx, nvy, nvz, nvc;
+  double dx, dy, dz, dc;
+  double o1, o2, o3, o4;
+  double a4m, a4sqr, a8, a12, b3, c3;
+  // parallel beam: view-constant footprint
+  double pb_rho0, pb_hu, pb_hv, pb_su, pb_sv, pb_hsu, pb_hsv;
+  // time interpolation: the view sees w0*x[t0] + w1*x[t1]
+  double w0, w1;
+  int t0, t1;
+  // Long & Fessler: cos/sin of beta, detector offset in pixels, object
+  // origin (x0, y0) in the rotated frame
+  double lf_cb, lf_sb, lf_off, lf_x0, lf_y0;
+};
+
+// A contiguous run of views (in plan order) sharing the same (t0, t1).
+struct Segment {
+  int start;   // index into the view order array
+  int count;
+  int t0, t1;
+};
+
+// Everything a kernel needs, passed by value.
+struct DevPlan {
+  const ViewConst *views;
+  const int *order;        // plan order -> view index (sinogram slot)
+  const Segment *segs;
+  const void *table;       // cumulative footprint table (T), or nullptr
+  int *err;                // set to 1 when a voxel lies in the source plane
+  long m;                  // table size
+  double step, coord0;
+  int nx, ny, nz, nu, nv, nviews, nt;
+  long vox_per_phase;      // nx*ny*nz
+  double u_scl, v_scl;
+  double hu_init, hv_init, rho_init, hs0;
+  // Long & Fessler scalars
+  double lf_vs, lf_ps, lf_R, lf_scale, lf_size;
+};
+
+}  // namespace trtt
+
+// Host-side plan (opaque to C callers).
+struct trtt_hip_plan {
+  int kind = 0;
+  int precision = TRTT_HIP_F32;
+  int device = 0;
+  bool table = false;
+  bool quirk_last_bin = false;
+  long nx = 0, ny = 0, nz = 0, nt = 1, nu = 1, nv = 1, nviews = 0;
+  double s_scl = 1.0, u_scl = 1.0, v_scl = 1.0;
+  int s_deg = 3;
+  std::vector<trtt::ViewConst> h_views;
+  std::vector<int> h_order;
+  std::vector<trtt::Segment> h_segs;
+  std::vector<int> h_seg_begin;        // per back-projection block row
+  trtt::ViewConst *d_views = nullptr;
+  int *d_order = nullptr;
+  trtt::Segment *d_segs = nullptr;
+  int *d_seg_begin = nullptr;
+  void *d_table = nullptr;
+  int *d_err = nullptr;
+  long m = 0;
+  double step = 0.0, coord0 = 0.0;
+  trtt::DevPlan dev{};
+};

@@ -1,0 +1,747 @@
+// gfx950 kernels of the B-spline footprint projectors and of the Long &
+// Fessler separable-footprint projector.
+//
+// Work decomposition.  A workgroup of 256 threads owns a tile of voxels and
+// walks a list of views.  For each view it
+//   1. evaluates the footprint of each of its voxels (FP64 detector
+//      geometry in the reference's operation order, so windows and
+//      tabulated-CDF indices are those of the C plugin),
+//   2. reduces the union of the voxel windows over the tile (wave shuffles
+//      plus one LDS exchange),
+//   3. forward: scatters separable weights into that detector window held in
+//      LDS, then flushes the window to HBM with one atomic add per non-zero
+//      pixel; backward: stages the window from HBM into LDS with coalesced
+//      row loads and gathers from LDS.
+// A window larger than the LDS budget falls back to direct global atomics /
+// loads for that (tile, view).
+#include <climits>
+
+#include "trtt_core.h"
+
+namespace trtt {
+
+extern __shared__ __align__(16) unsigned char g_smem[];
+
+constexpr int kThreads = 256;
+constexpr int kMaxW = 12;          // weights held in registers per chunk
+
+// ---------------------------------------------------------------------------
+// Closed-form cumulative integral of the centred cubic B-spline
+// (trtt_3D_projectors_cubic.c:117-138): F(x) = 1/2 + sign(x) G(|x|),
+// G(t) = t(2/3 + t^2(t/8 - 1/3)) on [0,1], 1/2 - (2-t)^4/24 on [1,2].
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T cdf3(T x) {
+  T t = fabs(x);
+  t = t < T(2) ? t : T(2);
+  const T t2 = t * t;
+  const T g1 = t * (T(2.0 / 3.0) + t2 * (t * T(0.125) - T(1.0 / 3.0)));
+  const T s = T(2) - t;
+  const T s2 = s * s;
+  const T g2 = T(0.5) - s2 * s2 * T(1.0 / 24.0);
+  const T g = t > T(1) ? g2 : g1;
+  return x < T(0) ? T(0.5) - g : T(0.5) + g;
+}
+
+// Footprint of one voxel on one view.
+struct Foot {
+  double uc, vc;       // projected centre (pixel-bound coordinates)
+  double hu, hv;       // footprint scales (pixels)
+  double su, sv;       // normalisations (1/h, or 1/(h*step) for 2-D tables)
+  double rho0;         // amplitude
+  int ul, ur, vl, vr;  // clamped window [ul,ur) x [vl,vr)
+};
+
+// Window clamp of trtt_3D_projectors.c:1293-1307; the 2-D closed-form
+// projectors clamp the right bound to n-1 (trtt_2D_projectors_cubic.c:256-258).
+template <bool QUIRK>
+__device__ __forceinline__ void clamp_window(double c, double hs, int n, int &lo, int &hi) {
+#pragma clang fp contract(off)
+  const double l = c - hs, r = c + hs;
+  const double nm1 = double(n - 1);
+  lo = l < 0.0 ? 0 : (l <= nm1 ? int(floor(l)) : n - 1);
+  if (!QUIRK) {
+    hi = r < 0.0 ? 0 : (r <= nm1 ? int(floor(r)) + 1 : n);
+  } else {
+    hi = r < 0.0 ? 0 : (r <= double(n - 2) ? int(floor(r)) + 1 : n - 1);
+  }
+}
+
+// Per-voxel footprint (FP64).  Returns false when the voxel lies in the
+// source plane (the reference's TRTT_FAILURE case).
+template <int KIND, bool TABLE, bool QUIRK>
+__device__ __forceinline__ bool voxel_footprint(const DevPlan &P, const ViewConst &g, int i, int j,
+                                                int k, Foot &f) {
+#pragma clang fp contract(off)
+  const double di = double(i), dj = double(j), dk = double(k);
+  constexpr bool CONE = (KIND == CB3D || KIND == FB2D);
+  constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
+  if (CONE) {
+    const double wk = ((g.o3 * dk + g.o4) + g.o2 * dj) + g.o1 * di;
+    const double num_v = ((g.nvz * dk + g.nvc) + g.nvy * dj) + g.nvx * di;
+    const double den = ((g.dz * dk + g.dc) + g.dy * dj) + g.dx * di;
+    const double q = g.a4m / den;
+    f.vc = q * num_v + g.b3;
+    if (!(wk != 0.0)) return false;
+    const double magn = g.a4m / wk;
+    const double vca8 = (f.vc - g.a8) * P.v_scl;
+    const double vca8sqr = vca8 * vca8;
+    const double dv = sqrt(1.0 + (vca8sqr / g.a4sqr));
+    if (!IS2D) {
+      const double num_u = ((g.nuz * dk + g.nuc) + g.nuy * dj) + g.nux * di;
+      f.uc = q * num_u + g.c3;
+      const double uca12 = (f.uc - g.a12) * P.u_scl;
+      const double du = sqrt(1.0 + ((uca12 * uca12) / (g.a4sqr + vca8sqr)));
+      f.rho0 = ((P.rho_init * (magn * magn)) * du) * dv;
+      f.hu = (P.hu_init * magn) * du;
+      f.su = 1.0 / f.hu;
+      f.hv = (P.hv_init * magn) * dv;
+      f.sv = 1.0 / f.hv;
+      clamp_window<false>(f.uc, P.hs0 * f.hu, P.nu, f.ul, f.ur);
+      clamp_window<false>(f.vc, P.hs0 * f.hv, P.nv, f.vl, f.vr);
+    } else {
+      f.uc = 0.5;
+      f.rho0 = (P.rho_init * magn) * dv;
+      f.hv = (P.hv_init * magn) * dv;
+      f.sv = TABLE ? 1 / (f.hv * P.step) : 1.0 / f.hv;
+      f.hu = 1.0;
+      f.su = 1.0;
+      f.ul = 0;
+      f.ur = 1;
+      clamp_window<QUIRK>(f.vc, P.hs0 * f.hv, P.nv, f.vl, f.vr);
+    }
+  } else {
+    f.vc = ((g.nvz * dk + g.nvc) + g.nvy * dj) + g.nvx * di;
+    f.rho0 = g.pb_rho0;
+    f.hv = g.pb_hv;
+    f.sv = g.pb_sv;
+    if (!IS2D) {
+      f.uc = ((g.nuz * dk + g.nuc) + g.nuy * dj) + g.nux * di;
+      f.hu = g.pb_hu;
+      f.su = g.pb_su;
+      clamp_window<false>(f.uc, g.pb_hsu, P.nu, f.ul, f.ur);
+      clamp_window<false>(f.vc, g.pb_hsv, P.nv, f.vl, f.vr);
+    } else {
+      f.uc = 0.5;
+      f.hu = 1.0;
+      f.su = 1.0;
+      f.ul = 0;
+      f.ur = 1;
+      clamp_window<QUIRK>(f.vc, g.pb_hsv, P.nv, f.vl, f.vr);
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ long table_index(double x, long m) {
+  double c = x >= 0.0 ? x : 0.0;
+  c = c <= double(m - 1) ? c : double(m - 1);
+  return long(floor(c));
+}
+
+// Streaming evaluator of one footprint axis: value(l) = F(x_{l+1}) - F(x_l)
+// over pixels [lo, lo+n).  Closed form: x_l = (lo-c)*s + l*s in T.  Table:
+// the reference's FP64 start expressions (trtt_3D_projectors.c:275-302 for
+// 3-D, trtt_2D_projectors.c:92-108 for 2-D) and floor() lookups.
+template <typename T, bool TABLE, bool TAB2D>
+struct AxisFoot {
+  double x0, s;  // table coordinate start and step (FP64)
+  T xf0, sf;     // closed-form start and step
+  T prev;
+  const T *cum;
+  long m;
+
+  __device__ __forceinline__ void init(const DevPlan &P, int lo, double c, double h, double scl) {
+#pragma clang fp contract(off)
+    if (TABLE) {
+      cum = static_cast<const T *>(P.table);
+      m = P.m;
+      if (TAB2D) {
+        s = scl;
+        x0 = (double(lo) - (c + P.coord0 * h)) * s;
+      } else {
+        s = scl / P.step;
+        x0 = ((double(lo) - c) - h * P.coord0) * s;
+      }
+      prev = cum[table_index(x0, m)];
+    } else {
+      xf0 = T((double(lo) - c) * scl);
+      sf = T(scl);
+      prev = cdf3<T>(xf0);
+    }
+  }
+  // weight of pixel lo+l (must be called with l = 0, 1, 2, ... in order)
+  __device__ __forceinline__ T next(int l) {
+#pragma clang fp contract(off)
+    T nxt;
+    if (TABLE) {
+      nxt = cum[table_index(x0 + double(l + 1) * s, m)];
+    } else {
+      nxt = cdf3<T>(xf0 + T(l + 1) * sf);
+    }
+    const T w = nxt - prev;
+    prev = nxt;
+    return w;
+  }
+};
+
+__device__ __forceinline__ int wave_min(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// Union of the voxel windows of the workgroup.  `red` holds 4 ints per wave.
+__device__ __forceinline__ void block_window(int &ul, int &ur, int &vl, int &vr, int *red) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  ul = wave_min(ul);
+  vl = wave_min(vl);
+  ur = wave_max(ur);
+  vr = wave_max(vr);
+  if (lane == 0) {
+    red[4 * w + 0] = ul;
+    red[4 * w + 1] = vl;
+    red[4 * w + 2] = ur;
+    red[4 * w + 3] = vr;
+  }
+  __syncthreads();
+  ul = red[0];
+  vl = red[1];
+  ur = red[2];
+  vr = red[3];
+#pragma unroll
+  for (int q = 1; q < kThreads / 64; ++q) {
+    ul = min(ul, red[4 * q + 0]);
+    vl = min(vl, red[4 * q + 1]);
+    ur = max(ur, red[4 * q + 2]);
+    vr = max(vr, red[4 * q + 3]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T *p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// Tile of TX x TY x TZ voxels; each of the 256 threads owns NPT voxels
+// stacked along k.
+template <int TX, int TY, int TZ>
+struct Tile {
+  static constexpr int NPT = (TX * TY * TZ) / kThreads;
+  static constexpr int KSTEP = kThreads / (TX * TY);
+  int i, j, k0;
+  __device__ __forceinline__ void init(const DevPlan &P) {
+    const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty;
+    const int bz = b / nty;
+    const int lid = threadIdx.x;
+    i = bx * TX + lid % TX;
+    j = by * TY + (lid / TX) % TY;
+    k0 = bz * TZ + lid / (TX * TY);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Forward: sino[view] += A_view (w0 x[t0] + w1 x[t1])
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, bool TABLE, bool QUIRK, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__restrict__ vol,
+                                                       T *__restrict__ sino, int views_per_block,
+                                                       int lds_cap) {
+  constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
+  using TL = Tile<TX, TY, TZ>;
+  T *reg = reinterpret_cast<T *>(g_smem);
+  __shared__ int red[4 * (kThreads / 64)];
+  TL tile;
+  tile.init(P);
+  const long npix = long(P.nu) * P.nv;
+
+  for (int vv = 0; vv < views_per_block; ++vv) {
+    const int view = blockIdx.y * views_per_block + vv;
+    if (view >= P.nviews) break;
+    const ViewConst &g = P.views[view];
+    T *out = sino + long(view) * npix;
+
+    Foot f[TL::NPT];
+    T val[TL::NPT];
+    bool act[TL::NPT];
+    int ul = INT_MAX, vl = INT_MAX, ur = INT_MIN, vr = INT_MIN;
+#pragma unroll
+    for (int q = 0; q < TL::NPT; ++q) {
+      const int k = tile.k0 + q * TL::KSTEP;
+      act[q] = tile.i < P.nx && tile.j < P.ny && k < P.nz;
+      val[q] = T(0);
+      if (act[q]) {
+        if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, tile.j, k, f[q])) {
+          atomicOr(P.err, 1);
+          act[q] = false;
+          continue;
+        }
+        const long idx = (long(k) * P.ny + tile.j) * P.nx + tile.i;
+        T x = T(g.w0) * vol[long(g.t0) * P.vox_per_phase + idx];
+        if (g.w1 != 0.0) x += T(g.w1) * vol[long(g.t1) * P.vox_per_phase + idx];
+        val[q] = x * T(f[q].rho0);
+        act[q] = f[q].ul < f[q].ur && f[q].vl < f[q].vr;
+        if (act[q]) {
+          ul = min(ul, f[q].ul);
+          ur = max(ur, f[q].ur);
+          vl = min(vl, f[q].vl);
+          vr = max(vr, f[q].vr);
+        }
+      }
+    }
+    block_window(ul, ur, vl, vr, red);
+    const bool any = ur > ul && vr > vl;
+    const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
+    const bool in_lds = any && long(RW) * RH <= lds_cap;
+    if (in_lds) {
+      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) reg[idx] = T(0);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < TL::NPT; ++q) {
+      if (!act[q]) continue;
+      const Foot &F = f[q];
+      const int npu = F.ur - F.ul, npv = F.vr - F.vl;
+      for (int l0 = 0; l0 < npu; l0 += kMaxW) {
+        T wu[kMaxW];
+        AxisFoot<T, TABLE, IS2D> au;
+        if (!IS2D) {
+          au.init(P, F.ul + l0, F.uc, F.hu, F.su);
+#pragma unroll
+          for (int l = 0; l < kMaxW; ++l) wu[l] = (l0 + l < npu) ? au.next(l) : T(0);
+        } else {
+#pragma unroll
+          for (int l = 0; l < kMaxW; ++l) wu[l] = l == 0 ? T(1) : T(0);
+        }
+        AxisFoot<T, TABLE, IS2D> av;
+        av.init(P, F.vl, F.vc, F.hv, F.sv);
+        for (int mm = 0; mm < npv; ++mm) {
+          const T c = av.next(mm) * val[q];
+          if (in_lds) {
+            T *row = reg + (F.vl + mm - vl) * RW + (F.ul + l0 - ul);
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l)
+              if (l0 + l < npu) atomicAdd(row + l, c * wu[l]);
+          } else {
+            T *row = out + long(F.vl + mm) * P.nu + (F.ul + l0);
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l)
+              if (l0 + l < npu) atomic_add(row + l, c * wu[l]);
+          }
+        }
+      }
+    }
+    if (in_lds) {
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) {
+        const T v = reg[idx];
+        if (v != T(0)) {
+          const int r = idx / RW, c = idx - r * RW;
+          atomic_add(out + long(vl + r) * P.nu + (ul + c), v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward: x[t0] += w0 A' y, x[t1] += w1 A' y over the views of the
+// segments [seg_begin[blockIdx.y], seg_begin[blockIdx.y+1]).
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, bool TABLE, bool QUIRK, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kThreads) void fp_backward(DevPlan P, const T *__restrict__ sino,
+                                                        T *__restrict__ vol,
+                                                        const int *__restrict__ seg_begin,
+                                                        int atomic_out, int lds_cap) {
+  constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
+  using TL = Tile<TX, TY, TZ>;
+  T *reg = reinterpret_cast<T *>(g_smem);
+  __shared__ int red[4 * (kThreads / 64)];
+  TL tile;
+  tile.init(P);
+  const long npix = long(P.nu) * P.nv;
+  const int s_begin = seg_begin[blockIdx.y], s_end = seg_begin[blockIdx.y + 1];
+
+  for (int s = s_begin; s < s_end; ++s) {
+    const Segment seg = P.segs[s];
+    T acc0[TL::NPT], acc1[TL::NPT];
+#pragma unroll
+    for (int q = 0; q < TL::NPT; ++q) acc0[q] = acc1[q] = T(0);
+
+    for (int n = 0; n < seg.count; ++n) {
+      const int view = P.order[seg.start + n];
+      const ViewConst &g = P.views[view];
+      const T *in = sino + long(view) * npix;
+      Foot f[TL::NPT];
+      bool act[TL::NPT];
+      int ul = INT_MAX, vl = INT_MAX, ur = INT_MIN, vr = INT_MIN;
+#pragma unroll
+      for (int q = 0; q < TL::NPT; ++q) {
+        const int k = tile.k0 + q * TL::KSTEP;
+        act[q] = tile.i < P.nx && tile.j < P.ny && k < P.nz;
+        if (act[q]) {
+          if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, tile.j, k, f[q])) {
+            atomicOr(P.err, 1);
+            act[q] = false;
+            continue;
+          }
+          act[q] = f[q].ul < f[q].ur && f[q].vl < f[q].vr;
+          if (act[q]) {
+            ul = min(ul, f[q].ul);
+            ur = max(ur, f[q].ur);
+            vl = min(vl, f[q].vl);
+            vr = max(vr, f[q].vr);
+          }
+        }
+      }
+      block_window(ul, ur, vl, vr, red);
+      const bool any = ur > ul && vr > vl;
+      const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
+      const bool in_lds = any && long(RW) * RH <= lds_cap;
+      if (in_lds) {
+        for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) {
+          const int r = idx / RW, c = idx - r * RW;
+          reg[idx] = in[long(vl + r) * P.nu + (ul + c)];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int q = 0; q < TL::NPT; ++q) {
+        if (!act[q]) continue;
+        const Foot &F = f[q];
+        const int npu = F.ur - F.ul, npv = F.vr - F.vl;
+        T sum = T(0);
+        for (int l0 = 0; l0 < npu; l0 += kMaxW) {
+          T wu[kMaxW];
+          AxisFoot<T, TABLE, IS2D> au;
+          if (!IS2D) {
+            au.init(P, F.ul + l0, F.uc, F.hu, F.su);
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l) wu[l] = (l0 + l < npu) ? au.next(l) : T(0);
+          } else {
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l) wu[l] = l == 0 ? T(1) : T(0);
+          }
+          AxisFoot<T, TABLE, IS2D> av;
+          av.init(P, F.vl, F.vc, F.hv, F.sv);
+          for (int mm = 0; mm < npv; ++mm) {
+            const T c = av.next(mm);
+            T rs = T(0);
+            if (in_lds) {
+              const T *row = reg + (F.vl + mm - vl) * RW + (F.ul + l0 - ul);
+#pragma unroll
+              for (int l = 0; l < kMaxW; ++l)
+                if (l0 + l < npu) rs += wu[l] * row[l];
+            } else {
+              const T *row = in + long(F.vl + mm) * P.nu + (F.ul + l0);
+#pragma unroll
+              for (int l = 0; l < kMaxW; ++l)
+                if (l0 + l < npu) rs += wu[l] * row[l];
+            }
+            sum += c * rs;
+          }
+        }
+        const T contrib = sum * T(F.rho0);
+        if (seg.t0 == seg.t1) {
+          acc0[q] += T(g.w0 + g.w1) * contrib;
+        } else {
+          acc0[q] += T(g.w0) * contrib;
+          acc1[q] += T(g.w1) * contrib;
+        }
+      }
+      __syncthreads();
+    }
+    // flush this segment's accumulators (each voxel is owned by one thread)
+#pragma unroll
+    for (int q = 0; q < TL::NPT; ++q) {
+      const int k = tile.k0 + q * TL::KSTEP;
+      if (!(tile.i < P.nx && tile.j < P.ny && k < P.nz)) continue;
+      const long idx = (long(k) * P.ny + tile.j) * P.nx + tile.i;
+      T *d0 = vol + long(seg.t0) * P.vox_per_phase + idx;
+      if (atomic_out) {
+        if (acc0[q] != T(0)) atomic_add(d0, acc0[q]);
+      } else {
+        *d0 += acc0[q];
+      }
+      if (seg.t0 != seg.t1) {
+        T *d1 = vol + long(seg.t1) * P.vox_per_phase + idx;
+        if (atomic_out) {
+          if (acc1[q] != T(0)) atomic_add(d1, acc1[q]);
+        } else {
+          *d1 += acc1[q];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Long & Fessler separable footprint (LongFessler2D.c:41-141, :254-355)
+// ---------------------------------------------------------------------------
+struct LFFoot {
+  double t[4];
+  double amp;
+  int kf, kl;  // inclusive bin range
+};
+
+__device__ __forceinline__ double lf_ramp(double a, double b, double c) {
+  return a < b ? (b + a) * (b - a) / (2.0 * c) : 0.0;
+}
+__device__ __forceinline__ double lf_box(double a, double b) { return a < b ? b - a : 0.0; }
+
+__device__ __forceinline__ void lf_footprint(const DevPlan &P, const ViewConst &g, int i1, int i2,
+                                             LFFoot &f) {
+#pragma clang fp contract(off)
+  const double vs = P.lf_vs;
+  const double y = g.lf_y0 + double(i2) * vs, ya = y - 0.5 * vs, yb = y + 0.5 * vs;
+  const double x = g.lf_x0 + double(i1) * vs, xa = x - 0.5 * vs, xb = x + 0.5 * vs;
+  const double cb = g.lf_cb, sb = g.lf_sb, R = P.lf_R;
+  const double cx[4] = {xa, xb, xa, xb}, cy[4] = {ya, ya, yb, yb};
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    f.t[c] = P.lf_scale * (cy[c] * sb + cx[c] * cb) / (R - (cy[c] * cb - cx[c] * sb)) + g.lf_off;
+  // the 5-comparator network of LongFessler2D.c:103-109
+#define LF_SORT(a, b)                 \
+  if (f.t[a] > f.t[b]) {              \
+    const double tmp = f.t[a];        \
+    f.t[a] = f.t[b];                  \
+    f.t[b] = tmp;                     \
+  }
+  LF_SORT(0, 1) LF_SORT(2, 3) LF_SORT(0, 2) LF_SORT(1, 3) LF_SORT(1, 2)
+#undef LF_SORT
+  const double qx = x + R * sb, qy = y - R * cb;
+  const double q = fabs(qy) > fabs(qx) ? qx / qy : qy / qx;
+  f.amp = sqrt(1.0 + q * q) * (vs * P.lf_ps);
+  const double lo = floor(f.t[0] - 0.5 * P.lf_size);
+  const double hi = ceil(f.t[3] + 0.5 * P.lf_size);
+  f.kf = lo < 0.0 ? 0 : (lo > double(P.nv) ? P.nv : int(lo));
+  f.kl = hi >= double(P.nv) ? P.nv - 1 : (hi < -1.0 ? -1 : int(hi));
+}
+
+__device__ __forceinline__ double lf_gamma(const LFFoot &f, int k, double size) {
+#pragma clang fp contract(off)
+  const double s1 = double(k) - 0.5 * size, s2 = s1 + size;
+  const double *t = f.t;
+  return lf_ramp((s1 >= t[0] ? s1 : t[0]) - t[0], (s2 <= t[1] ? s2 : t[1]) - t[0], t[1] - t[0]) +
+         lf_box(s1 >= t[1] ? s1 : t[1], s2 <= t[2] ? s2 : t[2]) +
+         lf_ramp(t[3] - (s2 <= t[3] ? s2 : t[3]), t[3] - (s1 >= t[2] ? s1 : t[2]), t[3] - t[2]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void lf_forward(DevPlan P, const T *__restrict__ img,
+                                                       T *__restrict__ sino, int views_per_block,
+                                                       int lds_cap) {
+  T *reg = reinterpret_cast<T *>(g_smem);
+  __shared__ int red[4 * (kThreads / 64)];
+  Tile<16, 16, 1> tile;
+  tile.init(P);
+  const bool act0 = tile.i < P.nx && tile.j < P.ny;
+  for (int vv = 0; vv < views_per_block; ++vv) {
+    const int view = blockIdx.y * views_per_block + vv;
+    if (view >= P.nviews) break;
+    const ViewConst &g = P.views[view];
+    T *out = sino + long(view) * P.nv;
+    LFFoot f;
+    T val = T(0);
+    bool act = act0;
+    int lo = INT_MAX, hi = INT_MIN, d0 = 0, d1 = 0;
+    if (act) {
+      lf_footprint(P, g, tile.i, tile.j, f);
+      val = img[long(tile.j) * P.nx + tile.i];
+      act = f.kf <= f.kl;
+      if (act) {
+        lo = f.kf;
+        hi = f.kl + 1;
+      }
+    }
+    block_window(lo, hi, d0, d1, red);
+    const int RW = hi > lo ? hi - lo : 0;
+    const bool in_lds = RW > 0 && RW <= lds_cap;
+    if (in_lds) {
+      for (int idx = threadIdx.x; idx < RW; idx += kThreads) reg[idx] = T(0);
+      __syncthreads();
+    }
+    if (act) {
+      const double a = f.amp * double(val);
+      for (int k = f.kf; k <= f.kl; ++k) {
+        const T c = T(lf_gamma(f, k, P.lf_size) * a);
+        if (in_lds)
+          atomicAdd(reg + (k - lo), c);
+        else
+          atomic_add(out + k, c);
+      }
+    }
+    if (in_lds) {
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < RW; idx += kThreads) {
+        const T v = reg[idx];
+        if (v != T(0)) atomic_add(out + lo + idx, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void lf_backward(DevPlan P, const T *__restrict__ sino,
+                                                        T *__restrict__ img, int views_per_block,
+                                                        int atomic_out) {
+  Tile<16, 16, 1> tile;
+  tile.init(P);
+  if (!(tile.i < P.nx && tile.j < P.ny)) return;
+  T acc = T(0);
+  for (int vv = 0; vv < views_per_block; ++vv) {
+    const int view = blockIdx.y * views_per_block + vv;
+    if (view >= P.nviews) break;
+    const ViewConst &g = P.views[view];
+    const T *in = sino + long(view) * P.nv;
+    LFFoot f;
+    lf_footprint(P, g, tile.i, tile.j, f);
+    double s = 0.0;
+    for (int k = f.kf; k <= f.kl; ++k) s += f.amp * lf_gamma(f, k, P.lf_size) * double(in[k]);
+    acc += T(s);
+  }
+  T *d = img + long(tile.j) * P.nx + tile.i;
+  if (atomic_out)
+    atomic_add(d, acc);
+  else
+    *d += acc;
+}
+
+// ---------------------------------------------------------------------------
+// Weighted residual and FP64 data-fidelity sum (Ytrtt4D_optim.i:418-446)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void weighted_residual(T *__restrict__ s,
+                                                              const T *__restrict__ y,
+                                                              const T *__restrict__ w, long n,
+                                                              double *__restrict__ fx) {
+  double local = 0.0;
+  for (long idx = long(blockIdx.x) * kThreads + threadIdx.x; idx < n;
+       idx += long(gridDim.x) * kThreads) {
+    const T d = s[idx] - y[idx];
+    const T r = w ? w[idx] * d : d;
+    local += double(r) * double(d);
+    s[idx] = r;
+  }
+  __shared__ double red[kThreads / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q];
+    atomicAdd(fx, 0.5 * t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (explicit instantiation table)
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, bool TABLE, bool QUIRK>
+hipError_t launch_fp(const DevPlan &P, int job, const void *in, void *out, const int *seg_begin,
+                     int nrows, int views_per_block, int atomic_out, hipStream_t st) {
+  constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
+  constexpr int TX = IS2D ? 16 : 8, TY = IS2D ? 16 : 8, TZ = IS2D ? 1 : 8;
+  const int lds_bytes = 32768;
+  const int lds_cap = lds_bytes / int(sizeof(T));
+  const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY, ntz = (P.nz + TZ - 1) / TZ;
+  const unsigned ntiles = unsigned(ntx) * nty * ntz;
+  if (job == 0) {
+    dim3 grid(ntiles, (P.nviews + views_per_block - 1) / views_per_block);
+    hipLaunchKernelGGL((fp_forward<T, KIND, TABLE, QUIRK, TX, TY, TZ>), grid, dim3(kThreads),
+                       lds_bytes, st, P, static_cast<const T *>(in), static_cast<T *>(out),
+                       views_per_block, lds_cap);
+  } else {
+    dim3 grid(ntiles, nrows);
+    hipLaunchKernelGGL((fp_backward<T, KIND, TABLE, QUIRK, TX, TY, TZ>), grid, dim3(kThreads),
+                       lds_bytes, st, P, static_cast<const T *>(in), static_cast<T *>(out),
+                       seg_begin, atomic_out, lds_cap);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fp_any(int kind, bool table, bool quirk, const DevPlan &P, int job,
+                         const void *in, void *out, const int *seg_begin, int nrows,
+                         int views_per_block, int atomic_out, hipStream_t st) {
+#define TRTT_CASE(K, TB, Q)                                                                \
+  if (kind == K && table == TB && quirk == Q)                                             \
+    return launch_fp<T, K, TB, Q>(P, job, in, out, seg_begin, nrows, views_per_block,      \
+                                  atomic_out, st);
+  TRTT_CASE(CB3D, false, false)
+  TRTT_CASE(CB3D, true, false)
+  TRTT_CASE(PB3D, false, false)
+  TRTT_CASE(PB3D, true, false)
+  TRTT_CASE(FB2D, false, true)
+  TRTT_CASE(FB2D, true, false)
+  TRTT_CASE(PB2D, false, true)
+  TRTT_CASE(PB2D, true, false)
+#undef TRTT_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const DevPlan &P, int job,
+                            const void *in, void *out, const int *seg_begin, int nrows,
+                            int views_per_block, int atomic_out, hipStream_t st) {
+  if (kind == LF2D) {
+    const int ntx = (P.nx + 15) / 16, nty = (P.ny + 15) / 16;
+    const int lds_bytes = 16384;
+    dim3 grid(unsigned(ntx) * nty, (P.nviews + views_per_block - 1) / views_per_block);
+    if (job == 0) {
+      if (f64)
+        hipLaunchKernelGGL(lf_forward<double>, grid, dim3(kThreads), lds_bytes, st, P,
+                           static_cast<const double *>(in), static_cast<double *>(out),
+                           views_per_block, lds_bytes / 8);
+      else
+        hipLaunchKernelGGL(lf_forward<float>, grid, dim3(kThreads), lds_bytes, st, P,
+                           static_cast<const float *>(in), static_cast<float *>(out),
+                           views_per_block, lds_bytes / 4);
+    } else {
+      if (f64)
+        hipLaunchKernelGGL(lf_backward<double>, grid, dim3(kThreads), 0, st, P,
+                           static_cast<const double *>(in), static_cast<double *>(out),
+                           views_per_block, atomic_out);
+      else
+        hipLaunchKernelGGL(lf_backward<float>, grid, dim3(kThreads), 0, st, P,
+                           static_cast<const float *>(in), static_cast<float *>(out),
+                           views_per_block, atomic_out);
+    }
+    return hipGetLastError();
+  }
+  if (f64)
+    return launch_fp_any<double>(kind, table, quirk, P, job, in, out, seg_begin, nrows,
+                                 views_per_block, atomic_out, st);
+  return launch_fp_any<float>(kind, table, quirk, P, job, in, out, seg_begin, nrows,
+                              views_per_block, atomic_out, st);
+}
+
+hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long n, double *fx,
+                           hipStream_t st) {
+  const long blocks_needed = (n + kThreads - 1) / kThreads;
+  const unsigned blocks = unsigned(blocks_needed < 4096 ? (blocks_needed > 0 ? blocks_needed : 1)
+                                                        : 4096);
+  if (f64)
+    hipLaunchKernelGGL(weighted_residual<double>, dim3(blocks), dim3(kThreads), 0, st,
+                       static_cast<double *>(s), static_cast<const double *>(y),
+                       static_cast<const double *>(w), n, fx);
+  else
+    hipLaunchKernelGGL(weighted_residual<float>, dim3(blocks), dim3(kThreads), 0, st,
+                       static_cast<float *>(s), static_cast<const float *>(y),
+                       static_cast<const float *>(w), n, fx);
+  return hipGetLastError();
+}
+
+}  // namespace trtt

@@ -1,0 +1,337 @@
+"""Host mirror of the TRTT operator layer over the gfx950 C ABI.
+
+Two levels, both calling ``libtrtt_hip.so`` only:
+
+* :class:`SingleProjector` mirrors the Yorick evaluators
+  ``trtt_single_3D_projector`` / ``_trtt_single_3D_projector``
+  (Ytrtt_3D_projectors.i:51-187), ``trtt_single_2D_projector``
+  (Ytrtt_2D_projectors.i:376-510) and ``trtt_single_Long_Fessler_2D_projector``
+  (Ytrtt_2D_projectors_Long_Fessler.i:134-280): one view, host FP64 arrays,
+  ``job`` 0 (direct) or 1 (transpose), a fresh output array per call, and
+  the same error codes (Ytrtt_error.i:47-62).
+* :class:`Projector` is the device-resident batched operator: every view of
+  every phase in one launch, on torch tensors already in HBM.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .geometry import (Geometry, XForm3D, detector_xi, footprint_table, object_xi,
+                       projector_coefs)
+
+# Geometry and model constants (Ytrtt_common.i:62-77)
+TRTT_SPLINE_DRIVEN = 1
+TRTT_DISTANCE_DRIVEN = 2
+TRTT_LONG_FESSLER = 3
+TRTT_PARALLEL_BEAM = 3
+TRTT_CONE_BEAM = 4
+TRTT_FAN_BEAM = 5
+
+# Error codes (Ytrtt_error.i:47-62)
+TRTT_ERR_OUT_OF_RANGE_INDEX = 5
+TRTT_ERR_NOT_TOMOBJ = 7
+TRTT_ERR_NOT_TOMDATA = 8
+TRTT_ERR_PROJECTOR_FAILURE = 11
+TRTT_ERR_UNSUPPORTED_JOB = 12
+TRTT_ERR_BAD_GEOMETRY = 13
+
+
+class TrttError(RuntimeError):
+    """A TRTT status error carrying the reference error code."""
+
+    def __init__(self, code: int, msg: str = ""):
+        super().__init__(f"TRTT error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_lib.dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+@dataclass
+class TomObj:
+    """Tomographic object (Ytrtt_tomobj.i:341-465): sizes, B-spline degree,
+    footprint table and the voxel->metric transform ``xi``."""
+
+    nx: int
+    ny: int
+    nz: int = 1
+    nt: int = 1
+    s_scl: float = 1.0
+    s_deg: int = 3
+    size_footprint: int = 10000
+    footprint: np.ndarray = field(init=False)
+    cum_footprint: np.ndarray = field(init=False)
+    coord_footprint: np.ndarray = field(init=False)
+    step_footprint: float = field(init=False)
+    xi: XForm3D = field(init=False)
+
+    def __post_init__(self):
+        fp, cum, coord, step = footprint_table(self.size_footprint, self.s_deg)
+        self.footprint, self.cum_footprint, self.coord_footprint = fp, cum, coord
+        self.step_footprint = step
+        self.xi = object_xi(self.nx, self.ny, self.nz, self.s_scl)
+
+
+@dataclass
+class TomData:
+    """One projection (Ytrtt_tomdata.i:210-288): detector sizes and scales, the
+    time index and the xos/xotilt/xsd transforms of its view."""
+
+    nu: int
+    nv: int
+    u_scl: float = 1.0
+    v_scl: float = 1.0
+    t_index: float = 0.0
+    xos: XForm3D = field(default_factory=XForm3D)
+    xotilt: XForm3D = field(default_factory=XForm3D)
+    xsd: XForm3D = field(default_factory=XForm3D)
+    theta: float = 0.0
+    Rsc: float | None = None
+    Rsd: float | None = None
+    voff: float = 0.0
+    xoff: float = 0.0
+    yoff: float = 0.0
+
+    @property
+    def xi(self) -> XForm3D:
+        return detector_xi(self.nu, self.nv, self.u_scl, self.v_scl)
+
+
+class SingleProjector:
+    """Evaluator of one view, calling the reference-named C ABI symbols.
+
+    ``model`` selects the footprint: ``"table"`` (trtt_PB/CB/FB*D, what the
+    Yorick layer uses) or ``"cubic"`` (the closed-form ``*_cubic`` variants).
+    """
+
+    def __init__(self, X: TomObj, Yk: TomData, mode: int, model: str = "table",
+                 is3d: bool | None = None, lf: bool = False):
+        if not isinstance(X, TomObj):
+            raise TrttError(TRTT_ERR_NOT_TOMOBJ)
+        if not isinstance(Yk, TomData):
+            raise TrttError(TRTT_ERR_NOT_TOMDATA)
+        if mode not in (TRTT_PARALLEL_BEAM, TRTT_FAN_BEAM, TRTT_CONE_BEAM):
+            raise TrttError(TRTT_ERR_BAD_GEOMETRY)
+        if not (0 <= Yk.t_index <= X.nt):
+            raise TrttError(TRTT_ERR_OUT_OF_RANGE_INDEX,
+                            "Incompatible TOMDATA 't_index' with dimension 'nt' of the TOMOBJ.")
+        self.X, self.Yk, self.mode, self.model, self.lf = X, Yk, mode, model, lf
+        self.is3d = (Yk.nu > 1) if is3d is None else is3d
+        self.xos_coefs, self.xsd_coefs = projector_coefs(X.xi, Yk.xi, Yk.xos, Yk.xotilt, Yk.xsd)
+        self.lib = _lib.load()
+        if lf:
+            self.lf_param = _lib.LF2DParameters(Yk.v_scl, Yk.v_scl, X.s_scl, Yk.Rsc, Yk.Rsd - Yk.Rsc,
+                                                Yk.xoff, Yk.yoff, X.nx, X.ny, Yk.nv)
+            self.beta = Yk.theta - math.pi / 2
+            self.offset = Yk.voff
+
+    def __call__(self, x, job: int = 0) -> np.ndarray:
+        X, Yk = self.X, self.Yk
+        if job not in (0, 1):
+            raise TrttError(TRTT_ERR_UNSUPPORTED_JOB)
+        x = _f64(x)
+        if self.lf:
+            out = np.zeros(Yk.nv if job == 0 else (X.ny, X.nx))
+            f = self.lib.lf2d_direct if job == 0 else self.lib.lf2d_adjoint
+            f(C.byref(self.lf_param), self.beta, self.offset, _ptr(x), _ptr(out), 1)
+            err = _lib.last_error()
+            if err:
+                raise TrttError(TRTT_ERR_PROJECTOR_FAILURE, err)
+            return out
+        if self.is3d:
+            out = np.zeros((Yk.nv, Yk.nu) if job == 0 else (X.nz, X.ny, X.nx))
+            pre = "CB3D" if self.mode != TRTT_PARALLEL_BEAM else "PB3D"
+            if self.model == "cubic":
+                r = getattr(self.lib, f"trtt_{pre}_cubic")(
+                    _ptr(out), _ptr(x), X.nx, X.ny, X.nz, X.s_scl, X.s_deg, Yk.nu, Yk.nv, Yk.u_scl,
+                    Yk.v_scl, _ptr(self.xos_coefs), _ptr(self.xsd_coefs), job)
+            else:
+                r = getattr(self.lib, f"trtt_{pre}")(
+                    _ptr(out), _ptr(x), X.nx, X.ny, X.nz, X.s_scl, X.s_deg, _ptr(X.footprint),
+                    _ptr(X.cum_footprint), X.size_footprint, X.step_footprint,
+                    _ptr(X.coord_footprint), Yk.nu, Yk.nv, Yk.u_scl, Yk.v_scl,
+                    _ptr(self.xos_coefs), _ptr(self.xsd_coefs), job)
+        else:
+            out = np.zeros(Yk.nv if job == 0 else (X.ny, X.nx))
+            pre = "FB2D" if self.mode != TRTT_PARALLEL_BEAM else "PB2D"
+            if self.model == "cubic":
+                r = getattr(self.lib, f"trtt_{pre}_cubic")(
+                    _ptr(out), _ptr(x), X.nx, X.ny, X.s_scl, X.s_deg, Yk.nv, Yk.v_scl,
+                    _ptr(self.xos_coefs), _ptr(self.xsd_coefs), job)
+            else:
+                r = getattr(self.lib, f"trtt_{pre}")(
+                    _ptr(out), _ptr(x), X.nx, X.ny, X.s_scl, X.s_deg, _ptr(X.footprint),
+                    _ptr(X.cum_footprint), X.size_footprint, X.step_footprint,
+                    _ptr(X.coord_footprint), Yk.nv, Yk.v_scl, _ptr(self.xos_coefs),
+                    _ptr(self.xsd_coefs), job)
+        if r == _lib.TRTT_FAILURE:
+            raise TrttError(TRTT_ERR_PROJECTOR_FAILURE, _lib.last_error())
+        return out
+
+
+def set_precision(fp64: bool) -> None:
+    """Precision of the reference-named symbols (default FP32)."""
+    if _lib.load().trtt_hip_set_precision(_lib.F64 if fp64 else _lib.F32) != 0:
+        raise RuntimeError(_lib.last_error())
+
+
+_KIND = {"parallel3d": _lib.PB3D, "cone": _lib.CB3D, "parallel2d": _lib.PB2D, "fan": _lib.FB2D}
+
+
+class Projector:
+    """Device-resident batched projector over every view of a geometry.
+
+    ``forward(vol, sino)`` accumulates ``sino[k] += A_k vol`` and
+    ``backward(sino, vol)`` accumulates ``vol += sum_k A_k' sino[k]`` on torch
+    tensors in HBM (float32 or float64 per ``precision``).  For a 4-D series
+    give ``nt`` and per-view ``t0, t1, w0, w1``: view k sees
+    ``w0[k] x[t0[k]] + w1[k] x[t1[k]]`` (Ytrtt_3D_projectors.i:254-375).
+    """
+
+    def __init__(self, geom: Geometry, model: str = "cubic", precision: str = "f32", nt: int = 1,
+                 t0=None, t1=None, w0=None, w1=None, size_footprint: int = 10000):
+        import torch  # device memory and streams only
+
+        self.torch = torch
+        self.geom = geom
+        self.lib = _lib.load()
+        self.precision = _lib.F64 if precision == "f64" else _lib.F32
+        self.dtype = torch.float64 if self.precision == _lib.F64 else torch.float32
+        self.nt = int(nt)
+        kind = _KIND[geom.kind]
+        self.kind = kind
+        xos, xsd = _f64(geom.xos), _f64(geom.xsd)
+        self._keep = [xos, xsd]
+        if model == "table":
+            _, cum, coord, step = footprint_table(size_footprint, geom.s_deg)
+            self._keep.append(cum)
+            cum_p, m, c0 = _ptr(cum), cum.size, float(coord[0])
+        else:
+            cum_p, m, step, c0 = None, 0, 0.0, 0.0
+
+        def iarr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            self._keep.append(a)
+            return a.ctypes.data_as(_lib.ip)
+
+        def darr(a):
+            if a is None:
+                return None
+            a = _f64(a)
+            self._keep.append(a)
+            return _ptr(a)
+
+        self.handle = self.lib.trtt_hip_plan_create(
+            kind, self.precision, geom.nx, geom.ny, geom.nz, self.nt, geom.s_scl, geom.s_deg,
+            geom.nu, geom.nv, geom.u_scl, geom.v_scl, _ptr(xos), _ptr(xsd), geom.nviews, cum_p, m,
+            step, c0, iarr(t0), iarr(t1), darr(w0), darr(w1))
+        if not self.handle:
+            raise RuntimeError("plan creation failed: " + _lib.last_error())
+
+    @property
+    def vol_shape(self):
+        g = self.geom
+        base = (g.nz, g.ny, g.nx) if g.is3d else (g.ny, g.nx)
+        return ((self.nt,) + base) if self.nt > 1 else base
+
+    @property
+    def sino_shape(self):
+        g = self.geom
+        return (g.nviews, g.nv, g.nu) if g.is3d else (g.nviews, g.nv)
+
+    def _check(self, t, shape):
+        torch = self.torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError("expected a device tensor")
+        if t.dtype != self.dtype or not t.is_contiguous() or tuple(t.shape) != tuple(shape):
+            raise ValueError(f"expected contiguous {self.dtype} of shape {shape}, got "
+                             f"{t.dtype} {tuple(t.shape)}")
+
+    def _stream(self, stream):
+        s = stream if stream is not None else self.torch.cuda.current_stream()
+        return C.c_void_p(s.cuda_stream)
+
+    def forward(self, vol, sino=None, stream=None):
+        if sino is None:
+            sino = self.torch.zeros(self.sino_shape, dtype=self.dtype, device=vol.device)
+        self._check(vol, self.vol_shape)
+        self._check(sino, self.sino_shape)
+        if self.lib.trtt_hip_forward(self.handle, C.c_void_p(vol.data_ptr()),
+                                     C.c_void_p(sino.data_ptr()), self._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return sino
+
+    def backward(self, sino, vol=None, stream=None):
+        if vol is None:
+            vol = self.torch.zeros(self.vol_shape, dtype=self.dtype, device=sino.device)
+        self._check(vol, self.vol_shape)
+        self._check(sino, self.sino_shape)
+        if self.lib.trtt_hip_backward(self.handle, C.c_void_p(sino.data_ptr()),
+                                      C.c_void_p(vol.data_ptr()), self._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return vol
+
+    def weighted_residual(self, sino, y, w, fx, stream=None):
+        """sino <- w*(sino-y) and fx += 0.5 sum w (sino-y)^2 (fx: 1-element float64)."""
+        wp = C.c_void_p(w.data_ptr()) if w is not None else None
+        if self.lib.trtt_hip_weighted_residual(self.handle, C.c_void_p(sino.data_ptr()),
+                                               C.c_void_p(y.data_ptr()), wp,
+                                               C.c_void_p(fx.data_ptr()),
+                                               self._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def error_flag(self) -> int:
+        return self.lib.trtt_hip_plan_error_flag(self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.trtt_hip_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LFProjector(Projector):
+    """Batched Long & Fessler projector (LongFessler2D.c) over many views."""
+
+    def __init__(self, params: _lib.LF2DParameters, beta, offset=None, precision: str = "f32"):
+        import torch
+
+        self.torch = torch
+        self.lib = _lib.load()
+        self.precision = _lib.F64 if precision == "f64" else _lib.F32
+        self.dtype = torch.float64 if self.precision == _lib.F64 else torch.float32
+        self.params = params
+        beta = _f64(beta)
+        offset = _f64(np.zeros_like(beta) if offset is None else offset)
+        self._keep = [beta, offset]
+        self.nt = 1
+        self.nviews = beta.size
+        self.handle = self.lib.trtt_hip_plan_create_lf2d(self.precision, C.byref(params), _ptr(beta),
+                                                         _ptr(offset), beta.size)
+        if not self.handle:
+            raise RuntimeError("plan creation failed: " + _lib.last_error())
+
+    @property
+    def vol_shape(self):
+        return (self.params.objectDimension2, self.params.objectDimension1)
+
+    @property
+    def sino_shape(self):
+        return (self.nviews, self.params.detectorDimension)
