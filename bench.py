@@ -1,0 +1,214 @@
+"""Cone-beam 256^3 forward+back projection throughput on one MI355X.
+
+Workload (the 3-D cone-beam configuration): 256^3 voxels of 1 mm on a cubic
+B-spline basis, 360 views over 360 deg, flat detector of 512 (transaxial, v)
+x 384 (axial, u) pixels at 1 mm, source-to-isocentre 1000 mm,
+source-to-detector 1500 mm, FP32 data, FP64 geometry.  One step = one
+forward projection of a volume over all 360 views plus one back projection
+of a sinogram over all 360 views (closed-form cubic footprint,
+trtt_CB3D_cubic semantics), inputs resident in HBM.
+
+Prints one JSON line with the throughput, the per-kernel roofline (HIP
+events on the launch stream) and the reference C plugin timed on the host
+cores over a bounded sample of views.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+NX = NY = NZ = 256
+NU, NV = 384, 512
+NVIEWS = 360
+RSC, RSD = 1000.0, 1500.0
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP32_PEAK = 157.3e12       # FLOP/s, MI355X FP32 vector
+
+
+def geometry():
+    from yoricktrtt_amd.geometry import cone_geometry
+
+    theta = np.arange(NVIEWS) * (2 * math.pi / NVIEWS)
+    return cone_geometry(NX, NY, NZ, 1.0, NU, NV, 1.0, 1.0, theta, RSC, RSD)
+
+
+def phantom(seed=3, n_ell=30):
+    """Seeded random ellipsoids (values in [0,1]) on the 256^3 grid."""
+    rng = np.random.default_rng(seed)
+    vol = np.zeros((NZ, NY, NX), dtype=np.float32)
+    half = np.array([(NX - 1) / 2, (NY - 1) / 2, (NZ - 1) / 2])
+    for _ in range(n_ell):
+        c = rng.uniform(-60, 60, size=3)
+        a = rng.uniform(8, 50, size=3)
+        mu = np.float32(rng.uniform(0, 1))
+        lo = np.maximum(np.floor(c - a + half).astype(int), 0)
+        hi = np.minimum(np.ceil(c + a + half).astype(int) + 1, [NX, NY, NZ])
+        zz, yy, xx = np.meshgrid(*(np.arange(lo[d], hi[d]) - half[d] for d in (2, 1, 0)),
+                                 indexing="ij")
+        q = ((xx - c[0]) / a[0]) ** 2 + ((yy - c[1]) / a[1]) ** 2 + ((zz - c[2]) / a[2]) ** 2
+        vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] += np.where(q <= 1.0, mu, np.float32(0))
+    return vol
+
+
+def algorithmic_bytes(esz=4):
+    """Per-view-operator bytes: each view reads the volume and writes its
+    projection (forward), or reads its projection and read-modify-writes the
+    volume (transpose)."""
+    vol = NX * NY * NZ * esz
+    det = NU * NV * esz
+    fwd = NVIEWS * (vol + det)
+    bwd = NVIEWS * (det + 2 * vol)
+    compulsory = 2 * vol + 2 * NVIEWS * det
+    return fwd, bwd, compulsory
+
+
+def cpu_reference_baseline(geo, threads):
+    """Reference C plugin (oracle/_ref) on `threads` host threads, each
+    projecting one view forward and one view back at the full size."""
+    from oracle.oracle import Reference
+
+    if not Reference.available():
+        return None
+    ref = Reference()
+    rng = np.random.default_rng(1)
+    vol = rng.uniform(size=(NZ, NY, NX))
+    sino = rng.uniform(size=(NV, NU))
+    views = np.linspace(0, NVIEWS - 1, threads).astype(int)
+    outs_f = [np.zeros((NV, NU)) for _ in range(threads)]
+    outs_b = [np.zeros((NZ, NY, NX)) for _ in range(threads)]
+
+    def work(t):
+        v = views[t]
+        ref.CB3D_cubic(outs_f[t], vol, NX, NY, NZ, 1.0, NU, NV, 1.0, 1.0, geo.xos[v], geo.xsd[v], 0)
+        ref.CB3D_cubic(outs_b[t], sino, NX, NY, NZ, 1.0, NU, NV, 1.0, 1.0, geo.xos[v], geo.xsd[v], 1)
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    dt = time.perf_counter() - t0
+    view_pairs_per_s = threads / dt
+    return {"value": view_pairs_per_s / NVIEWS, "unit": "pairs/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{threads} threads x (1 forward + 1 back view of trtt_CB3D_cubic, FP64, "
+                      f"256^3 -> 512x384), {dt:.1f} s wall, scaled to 360 views"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--model", default="cubic", choices=["cubic", "table"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("BENCH_CPU_THREADS", 16)))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from yoricktrtt_amd.projectors import Projector
+
+    dev = torch.device("cuda:0")
+    geo = geometry()
+    P = Projector(geo, model=args.model, precision="f32")
+    print(f"bench: plan ready ({NVIEWS} views)", file=sys.stderr, flush=True)
+    x = torch.from_numpy(phantom()).to(dev)
+    g = torch.Generator(device=dev).manual_seed(7)
+    y = torch.rand((NVIEWS, NV, NU), generator=g, device=dev, dtype=torch.float32)
+    sino = torch.zeros((NVIEWS, NV, NU), device=dev, dtype=torch.float32)
+    vol = torch.zeros((NZ, NY, NX), device=dev, dtype=torch.float32)
+
+    def step(evs=None):
+        sino.zero_()
+        vol.zero_()
+        if evs:
+            evs[0].record()
+        P.forward(x, sino)
+        if evs:
+            evs[1].record()
+        P.backward(y, vol)
+        if evs:
+            evs[2].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = np.mean([e[0].elapsed_time(e[1]) for e in ev])
+    bwd_ms = np.mean([e[1].elapsed_time(e[2]) for e in ev])
+    if P.error_flag() != 0:
+        raise RuntimeError("a voxel fell in the source plane")
+
+    ms_per_step = wall * 1e3 / args.steps
+    pairs_per_s = args.steps / wall
+    fwd_b, bwd_b, comp_b = algorithmic_bytes()
+    dom_is_bwd = bwd_ms >= fwd_ms
+    dom_ms = bwd_ms if dom_is_bwd else fwd_ms
+    dom_bytes = bwd_b if dom_is_bwd else fwd_b
+    achieved = dom_bytes / (dom_ms * 1e-3)
+    result = {
+        "metric": "cone 256^3 fwd+back pairs/s",
+        "value": round(pairs_per_s, 4),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (30 seeded ellipsoids; uniform random sinogram)",
+        "config": {"workload": "3D cone beam 256^3 x 1 mm, 360 views / 360 deg, 512x384 flat "
+                               "detector 1 mm, Rsc 1000 mm, Rsd 1500 mm, cubic footprint",
+                   "model": args.model},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "fp_backward" if dom_is_bwd else "fp_forward",
+            "achieved": round(achieved / 1e9, 1),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": None,
+            "bytes_model": "per-view-operator bytes (each view reads the volume; transpose "
+                           "read-modify-writes it)",
+        },
+        "kernels_ms": {"fp_forward": round(fwd_ms, 3), "fp_backward": round(bwd_ms, 3)},
+        "pair_fracs": {
+            "hbm_per_view_bytes": round((fwd_b + bwd_b) * pairs_per_s / HBM_PEAK, 4),
+            "hbm_compulsory_bytes": round(comp_b * pairs_per_s / HBM_PEAK, 6),
+        },
+    }
+    if not args.no_cpu:
+        print("bench: timing the reference C plugin on the host", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_reference_baseline(geo, args.cpu_threads)
+    else:
+        result["cpu_baseline"] = None
+    P.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
