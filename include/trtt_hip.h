@@ -178,6 +178,10 @@ int trtt_hip_weighted_residual(trtt_hip_plan_t *plan, void *sino, const void *y,
  * reference's TRTT_FAILURE case), 0 otherwise, -1 on error.  Synchronous. */
 int trtt_hip_plan_error_flag(trtt_hip_plan_t *plan);
 
+/* Kernel family a plan uses: 0 = general footprint kernels, W > 0 = the
+ * axial closed-form kernels with a footprint width bound of W pixels. */
+int trtt_hip_plan_path(const trtt_hip_plan_t *plan);
+
 long trtt_hip_plan_nviews(const trtt_hip_plan_t *plan);
 long trtt_hip_plan_volume_size(const trtt_hip_plan_t *plan);   /* nt*nz*ny*nx */
 long trtt_hip_plan_sino_size(const trtt_hip_plan_t *plan);     /* nviews*nv*nu */

@@ -194,6 +194,8 @@ def test_batched_plan_vs_oracle(lib, oracle, kind, model, prec):
     vol = rng.uniform(size=(nz, ny, nx))
     sino = rng.uniform(size=(V, nv, nu))
     P = Projector(geo, model=model, precision=prec, size_footprint=10000)
+    expect_axial = model == "cubic" and prec == "f32"
+    assert (P.path > 0) == expect_axial, P.path
     dt = torch.float64 if prec == "f64" else torch.float32
     fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
     bwd = P.backward(_dev(sino, dt)).double().cpu().numpy()
@@ -323,4 +325,72 @@ def test_weighted_residual(lib):
     P.weighted_residual(s, y, w, fx)
     assert abs(fx.item() - ref_fx) < 1e-6 * ref_fx
     assert rel_err(s.double().cpu().numpy(), ref_r.cpu().numpy()) < 1e-6
+    P.close()
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel3d"])
+def test_tilted_orbit_uses_general_kernels(lib, oracle, kind):
+    """A rotation axis tilted away from z is not axial: the general kernels
+    run and still match the oracle."""
+    from yoricktrtt_amd.geometry import (Geometry, XForm3D, detector_xi, object_xi,
+                                         projector_coefs, view_transforms_3d)
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("tilted", kind)
+    nx, ny, nz, nu, nv, V = 20, 18, 16, 30, 34, 5
+    oxi = object_xi(nx, ny, nz, 1.0)
+    dxi = detector_xi(nu, nv, 1.0, 1.0)
+    xs, ds = [], []
+    for th in np.linspace(0.2, 5.0, V):
+        Rsc, Rsd = (80.0, 130.0) if kind == "cone" else (0.0, 0.0)
+        xos, xot, xsd = view_transforms_3d(th, Rsc, Rsd)
+        tilt = XForm3D().rotate(math.cos(0.15), math.sin(0.15), 0.0, 0.0)
+        xos = XForm3D.combine(xos, tilt)
+        a, b = projector_coefs(oxi, dxi, xos, xot, xsd)
+        xs.append(a)
+        ds.append(b)
+    geo = Geometry(kind, nx, ny, nz, nu, nv, 1.0, 1.0, 1.0, np.array(xs), np.array(ds))
+    P = Projector(geo, model="cubic", precision="f32")
+    assert P.path == 0
+    rng = np.random.default_rng(4)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v],
+                         geo.xsd[v], kind == "cone", 0)
+        assert rel_err(fwd[v], rf) < 1e-5
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v],
+                    kind == "cone", 1)
+    assert rel_err(bwd, rb) < 1e-5
+    P.close()
+
+
+def test_axial_wide_footprints_and_truncation(lib, oracle):
+    """Large magnification (W = 13 kernels) and a detector smaller than the
+    projected volume (windows clipped at every edge, hulls at the borders)."""
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("wide footprints")
+    nx, ny, nz, nu, nv, V = 33, 29, 70, 41, 45, 6
+    us, vs = 1.0, 1.0
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, us, vs, np.linspace(0.0, 6.0, V), 70.0, 120.0,
+                        u_off=3.0, v_off=-2.0)
+    P = Projector(geo, model="cubic", precision="f32")
+    assert P.path == 13
+    rng = np.random.default_rng(8)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v],
+                         geo.xsd[v], True, 0)
+        assert rel_err(fwd[v], rf) < 1e-5
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v], geo.xsd[v], True, 1)
+    assert rel_err(bwd, rb) < 1e-5
     P.close()

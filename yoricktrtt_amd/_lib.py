@@ -54,6 +54,7 @@ _SIGS = {
     "trtt_hip_backward": ([vp, vp, vp, vp], I),
     "trtt_hip_weighted_residual": ([vp, vp, vp, vp, vp, vp], I),
     "trtt_hip_plan_error_flag": ([vp], I),
+    "trtt_hip_plan_path": ([vp], I),
     "trtt_hip_plan_nviews": ([vp], L),
     "trtt_hip_plan_volume_size": ([vp], L),
     "trtt_hip_plan_sino_size": ([vp], L),

@@ -1,0 +1,765 @@
+// Fast closed-form footprint kernels for "axial" views: geometries whose
+// rotation axis is the object z axis and the detector u axis (every circular
+// cone- or parallel-beam orbit built by the TRTT system builders).  There the
+// transaxial position vc, the magnification and the v footprint of a voxel
+// depend on its column (i, j) only, and uc is affine in k
+// (trtt_3D_projectors.c:1234-1261 with o3 = den_z = num_v_z = 0).
+//
+// Mapping: a wave owns one voxel column at a time with its 64 consecutive k
+// in the lanes.
+//  * The column's v weights are wave-uniform: lane L evaluates the CDF at
+//    the L-th v pixel bound, readlane broadcasts the differences.
+//  * Forward: the lanes accumulate the column's axial profile
+//    P[u] = sum_k x_k rho_k w_k(u) in wave-private LDS (distinct addresses
+//    per step, no collisions), then add the outer product P x cv into the
+//    workgroup's detector window in LDS; the window is flushed to HBM with
+//    one atomic per non-zero pixel.
+//  * Backward: the window is staged from HBM; the lanes first form
+//    Q[u] = sum_m cv[m] y[vl+m][u] for the column, then each lane gathers its
+//    voxel's axial weights against Q.
+// Voxels whose window leaves the column hull take a direct global path.
+#include <climits>
+
+#include "trtt_core.h"
+
+namespace trtt {
+
+extern __shared__ __align__(16) unsigned char g_smem[];
+
+namespace axial {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kKT = 64;  // k per tile (one per lane)
+
+// F(x) - 1/2 for the cubic B-spline CDF: sign(x) G(min(|x|, 2)).
+template <typename T>
+__device__ __forceinline__ T cdf3c(T x) {
+  T t = fabs(x);
+  t = t < T(2) ? t : T(2);
+  const T t2 = t * t;
+  const T g1 = t * (T(2.0 / 3.0) + t2 * (t * T(0.125) - T(1.0 / 3.0)));
+  const T s = T(2) - t;
+  const T s2 = s * s;
+  const T g2 = T(0.5) - s2 * s2 * T(1.0 / 24.0);
+  const T g = t > T(1) ? g2 : g1;
+  return x < T(0) ? -g : g;
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+
+template <typename T>
+__device__ __forceinline__ T bcast(T v, int lane);
+template <>
+__device__ __forceinline__ float bcast<float>(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+template <>
+__device__ __forceinline__ double bcast<double>(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+  return __builtin_bit_cast(double, (long long)(unsigned(lo)) | ((long long)hi << 32));
+}
+
+__device__ __forceinline__ void clampw(double c, double hs, int n, int &lo, int &hi) {
+  const double l = c - hs, r = c + hs;
+  const double nm1 = double(n - 1);
+  lo = l < 0.0 ? 0 : (l <= nm1 ? int(floor(l)) : n - 1);
+  hi = r < 0.0 ? 0 : (r <= nm1 ? int(floor(r)) + 1 : n);
+}
+
+// Column constants of one view.
+template <typename T>
+struct Col {
+  double vc;       // transaxial centre
+  double uc0, duc; // uc(k) = uc0 + duc k
+  T rho;           // rho0 / du
+  T hu0;           // hu / du
+  T inv_a;         // 1 / (a4^2 + vca8^2): du = sqrt(1 + uca12^2 inv_a) (0 for parallel)
+  T sv;            // 1 / hv
+  int vl, vr;      // v window
+  bool ok;
+};
+
+template <bool CONE, typename T>
+__device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int i, int j, Col<T> &c) {
+  const double di = double(i), dj = double(j);
+  if (CONE) {
+    const double wk = (g.o4 + g.o2 * dj) + g.o1 * di;
+    const double num_v = (g.nvc + g.nvy * dj) + g.nvx * di;
+    const double num_u = (g.nuc + g.nuy * dj) + g.nux * di;
+    const double den = (g.dc + g.dy * dj) + g.dx * di;
+    const double q = g.a4m / den;
+    c.ok = wk != 0.0;
+    c.vc = q * num_v + g.b3;
+    c.uc0 = q * num_u + g.c3;
+    c.duc = q * g.nuz;
+    const double magn = g.a4m / wk;
+    const double vca8 = (c.vc - g.a8) * P.v_scl;
+    const double a = g.a4sqr + vca8 * vca8;
+    const double dv = sqrt(1.0 + vca8 * vca8 / g.a4sqr);
+    const double hv = P.hv_init * magn * dv;
+    c.sv = T(1.0 / hv);
+    c.rho = T(P.rho_init * magn * magn * dv);
+    c.hu0 = T(P.hu_init * magn);
+    c.inv_a = T(1.0 / a);
+    clampw(c.vc, P.hs0 * hv, P.nv, c.vl, c.vr);
+  } else {
+    c.ok = true;
+    c.vc = (g.nvc + g.nvy * dj) + g.nvx * di;
+    c.uc0 = (g.nuc + g.nuy * dj) + g.nux * di;
+    c.duc = g.nuz;
+    c.sv = T(g.pb_sv);
+    c.rho = T(g.pb_rho0);
+    c.hu0 = T(g.pb_hu);
+    c.inv_a = T(0);
+    clampw(c.vc, g.pb_hsv, P.nv, c.vl, c.vr);
+  }
+}
+
+// Axial footprint of voxel k of a column.  The window bounds are taken
+// from the single-precision centre (an error there only moves a bound onto
+// a pixel of zero weight); the sub-pixel offset x0 uses the FP64 centre.
+template <typename T>
+struct Vox {
+  double uc;
+  T su, rho;
+  int ul, ur;
+  bool clipped;  // left bound clamped at the detector edge
+};
+
+__device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int &hi,
+                                         bool &clipped) {
+  const float l = c - hs, r = c + hs;
+  const float nm1 = float(n - 1);
+  lo = l < 0.0f ? 0 : (l <= nm1 ? int(floorf(l)) : n - 1);
+  hi = r < 0.0f ? 0 : (r <= nm1 ? int(floorf(r)) + 1 : n);
+  clipped = !(l >= 0.0f && l <= nm1);
+}
+
+template <typename T>
+__device__ __forceinline__ void voxel(const DevPlan &P, const ViewConst &g, const Col<T> &c, int k,
+                                      Vox<T> &v) {
+  v.uc = c.uc0 + c.duc * double(k);
+  const T uca12 = T((v.uc - g.a12) * P.u_scl);
+  const T du = fsqrt(T(1) + uca12 * uca12 * c.inv_a);
+  const T hu = c.hu0 * du;
+  v.su = frcp(hu);
+  v.rho = c.rho * du;
+  clampw_f(float(v.uc), float(T(P.hs0) * hu), P.nu, v.ul, v.ur, v.clipped);
+}
+
+// Weights of pixels [ul, ul+W) (zero beyond ur).  An unclipped window starts
+// at least 2h left of the centre, where the CDF is exactly 0.
+template <int W, typename T>
+__device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[W]) {
+  const T x0 = T(double(v.ul) - v.uc) * v.su;
+  const int n = v.ur - v.ul;
+  T prev = T(-0.5);
+  if (v.clipped) prev = cdf3c<T>(x0);
+#pragma unroll
+  for (int l = 0; l < W; ++l) {
+    const T nx = cdf3c<T>(x0 + T(l + 1) * v.su);
+    w[l] = l < n ? nx - prev : T(0);
+    prev = nx;
+  }
+}
+
+// v weights of the column, broadcast from lanes.
+template <int W, typename T>
+__device__ __forceinline__ void vweights(const Col<T> &c, int lane, T (&cv)[W]) {
+  const T f = cdf3c<T>(T(double(c.vl + lane) - c.vc) * c.sv);
+  const int n = c.vr - c.vl;
+  T prev = bcast(f, 0);
+#pragma unroll
+  for (int m = 0; m < W; ++m) {
+    const T nx = bcast(f, m + 1);
+    cv[m] = m < n ? nx - prev : T(0);
+    prev = nx;
+  }
+}
+
+// Slow path for one voxel: arbitrary window sizes, direct global memory.
+template <bool FWD, typename T>
+__device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T val, T *out,
+                           const T *in, T &acc) {
+  const int npu = v.ur - v.ul, npv = c.vr - c.vl;
+  if (npu <= 0 || npv <= 0) return;
+  const T xu0 = T(double(v.ul) - v.uc) * v.su;
+  const T xv0 = T(double(c.vl) - c.vc) * c.sv;
+  T fv0 = cdf3c<T>(xv0);
+  T sum = T(0);
+  for (int m = 0; m < npv; ++m) {
+    const T fv1 = cdf3c<T>(xv0 + T(m + 1) * c.sv);
+    const T wv = fv1 - fv0;
+    fv0 = fv1;
+    T fu0 = cdf3c<T>(xu0);
+    for (int l = 0; l < npu; ++l) {
+      const T fu1 = cdf3c<T>(xu0 + T(l + 1) * v.su);
+      const T w = wv * (fu1 - fu0);
+      fu0 = fu1;
+      const long pix = long(c.vl + m) * P.nu + (v.ul + l);
+      if (FWD)
+        unsafeAtomicAdd(out + pix, w * val);
+      else
+        sum += w * in[pix];
+    }
+  }
+  if (!FWD) acc += sum * v.rho;
+}
+
+struct TileMap {
+  int bx, by, bz;
+  __device__ __forceinline__ void init(const DevPlan &P, int cx, int cy) {
+    const int ntx = (P.nx + cx - 1) / cx, nty = (P.ny + cy - 1) / cy;
+    int b = blockIdx.x;
+    bx = b % ntx;
+    b /= ntx;
+    by = b % nty;
+    bz = b / nty;
+  }
+};
+
+__device__ __forceinline__ void block_union(int &a, int &b, int &c, int &d, int *red) {
+  // a, c: min; b, d: max; values are wave-uniform on entry
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[4 * w + 0] = a;
+    red[4 * w + 1] = b;
+    red[4 * w + 2] = c;
+    red[4 * w + 3] = d;
+  }
+  __syncthreads();
+  a = red[0];
+  b = red[1];
+  c = red[2];
+  d = red[3];
+#pragma unroll
+  for (int q = 1; q < kWaves; ++q) {
+    a = min(a, red[4 * q + 0]);
+    b = max(b, red[4 * q + 1]);
+    c = min(c, red[4 * q + 2]);
+    d = max(d, red[4 * q + 3]);
+  }
+}
+
+// Column hull of the u windows over the k range [kf, kl] (uc is affine in k
+// and the window bounds are monotone in uc): windows at the two ends, padded
+// by one pixel.  Voxels are still checked against it.
+template <typename T>
+__device__ __forceinline__ void column_hull(const DevPlan &P, const ViewConst &g, const Col<T> &c,
+                                            int kf, int kl, int &lo, int &hi) {
+  Vox<T> a, b;
+  voxel(P, g, c, kf, a);
+  voxel(P, g, c, kl, b);
+  lo = max(0, min(a.ul, b.ul) - 1);
+  hi = min(P.nu, max(a.ur, b.ur) + 1);
+}
+
+// ---------------------------------------------------------------------------
+// Column table of a wave: lane L evaluates column L of the wave (geometry,
+// v window, u hull over the tile's k range); the column loop then broadcasts
+// column cc from lane cc with readlane.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ColLane {
+  Col<T> c;
+  int lo, hi;   // u hull of the column over [kf, kl]
+  int i, j;     // voxel column
+  bool live;    // inside the volume, wk != 0, non-empty windows
+  bool bad;     // wk == 0 (voxel in the source plane)
+};
+
+template <bool CONE, int CX, int NCOL, typename T>
+__device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &g, const TileMap &tm,
+                                             int wave, int lane, int kf, int kl, ColLane<T> &cl) {
+  const int col = wave * NCOL + lane;
+  cl.i = tm.bx * CX + col % CX;
+  cl.j = tm.by * (NCOL * kWaves / CX) + col / CX;
+  cl.live = false;
+  cl.bad = false;
+  cl.lo = cl.hi = 0;
+  if (lane < NCOL && cl.i < P.nx && cl.j < P.ny) {
+    column<CONE>(P, g, cl.i, cl.j, cl.c);
+    cl.bad = !cl.c.ok;
+    if (cl.c.ok && cl.c.vl < cl.c.vr) {
+      column_hull(P, g, cl.c, kf, kl, cl.lo, cl.hi);
+      cl.live = cl.lo < cl.hi;
+    }
+  }
+}
+
+__device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+template <typename T>
+__device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> &c, int &lo, int &hi,
+                                          int &i, int &j) {
+  c.vc = bcast(src.c.vc, cc);
+  c.uc0 = bcast(src.c.uc0, cc);
+  c.duc = bcast(src.c.duc, cc);
+  c.rho = bcast(src.c.rho, cc);
+  c.hu0 = bcast(src.c.hu0, cc);
+  c.inv_a = bcast(src.c.inv_a, cc);
+  c.sv = bcast(src.c.sv, cc);
+  c.vl = bcast_i(src.c.vl, cc);
+  c.vr = bcast_i(src.c.vr, cc);
+  c.ok = true;
+  lo = bcast_i(src.lo, cc);
+  hi = bcast_i(src.hi, cc);
+  i = bcast_i(src.i, cc);
+  j = bcast_i(src.j, cc);
+}
+
+__device__ __forceinline__ int wred_min(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ int wred_max(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// Block window: union over the live columns of the four waves.
+template <typename T>
+__device__ __forceinline__ void block_window(const ColLane<T> &cl, int *red, int &ul, int &ur,
+                                             int &vl, int &vr) {
+  ul = wred_min(cl.live ? cl.lo : INT_MAX);
+  ur = wred_max(cl.live ? cl.hi : INT_MIN);
+  vl = wred_min(cl.live ? cl.c.vl : INT_MAX);
+  vr = wred_max(cl.live ? cl.c.vr : INT_MIN);
+  block_union(ul, ur, vl, vr, red);
+}
+
+// ---------------------------------------------------------------------------
+// Forward.  Rounds of four columns (one per wave): each wave accumulates its
+// column's axial profile in wave-private LDS (plain read-modify-write when
+// the host has shown that the lanes of a wave hit distinct pixels at each
+// step, LDS atomics otherwise); after a barrier the 256 threads apply the
+// four outer products P x cv to the workgroup's detector window with each
+// thread owning distinct u, so the window needs no atomics either.
+// ---------------------------------------------------------------------------
+template <bool CONE, int W, int CX, int CY, typename T>
+__global__ __launch_bounds__(kThreads) void fwd(DevPlan P, const T *__restrict__ vol,
+                                                T *__restrict__ sino, int views_per_block,
+                                                int wcap, int pmax, int view_base,
+                                                int view_end, T *__restrict__ scratch,
+                                                int4 *__restrict__ hdr, int plain) {
+  constexpr int NCOL = CX * CY / kWaves;  // columns per wave
+  static_assert(NCOL <= 64, "one column per lane");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T *win = reinterpret_cast<T *>(g_smem);
+  T *profs = win + wcap;
+  T *prof = profs + wave * pmax;
+  __shared__ int red[4 * kWaves];
+  __shared__ int meta[kWaves][4];   // per wave: lo, vl, npv, span (span 0: nothing)
+  __shared__ T cvs[kWaves][W];
+  TileMap tm;
+  tm.init(P, CX, CY);
+  const int k = tm.bz * kKT + lane;
+  const bool kin = k < P.nz;
+  const int kf = tm.bz * kKT, kl = min(kf + kKT - 1, P.nz - 1);
+  const long npix = long(P.nu) * P.nv;
+  const long ntiles = gridDim.x;
+
+  for (int vv = 0; vv < views_per_block; ++vv) {
+    const int view = view_base + blockIdx.y * views_per_block + vv;
+    if (view >= view_end) break;
+    const ViewConst &g = P.views[view];
+    T *out = sino + long(view) * npix;
+    const long slot = long(view - view_base) * ntiles + blockIdx.x;
+
+    ColLane<T> cl;
+    column_table<CONE, CX, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
+    if (cl.bad) atomicOr(P.err, 1);
+    int ul, ur, vl, vr;
+    block_window(cl, red, ul, ur, vl, vr);
+    const bool any = ur > ul && vr > vl;
+    const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
+    const bool in_lds = any && RW * RH <= wcap;
+    if (in_lds)
+      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) win[idx] = T(0);
+    const unsigned long long live_mask = __ballot(cl.live);
+
+    for (int cc = 0; cc < NCOL; ++cc) {
+      // --- this wave's column cc: profile in LDS ---
+      int m_lo = 0, m_vl = 0, m_npv = 0, m_span = 0;
+      if (any && ((live_mask >> cc) & 1ull)) {
+        Col<T> c;
+        int lo, hi, i, j;
+        bcast_col(cl, cc, c, lo, hi, i, j);
+        const int span = hi - lo;
+        const int npv = c.vr - c.vl;
+        const bool use_prof = in_lds && span <= pmax && npv <= W;
+        T cv[W];
+        vweights<W>(c, lane, cv);
+        T val = T(0);
+        Vox<T> v;
+        bool act = false;
+        if (kin) {
+          voxel(P, g, c, k, v);
+          act = v.ul < v.ur;
+          // volume copy with k fastest: the 64 lanes read 64 consecutive words
+          const long idx = (long(j) * P.nx + i) * P.nz + k;
+          T x = T(g.w0) * vol[long(g.t0) * P.vox_per_phase + idx];
+          if (g.w1 != 0.0) x += T(g.w1) * vol[long(g.t1) * P.vox_per_phase + idx];
+          val = x * v.rho;
+        }
+        if (use_prof) {
+          for (int u = lane; u < span; u += 64) prof[u] = T(0);
+          __builtin_amdgcn_wave_barrier();
+          const bool fast = act && !v.clipped && v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W;
+          if (fast) {
+            T w[W];
+            uweights<W>(v, w);
+            T *pp = prof + (v.ul - lo);
+            if (plain) {
+#pragma unroll
+              for (int l = 0; l < W; ++l) {
+                if (l < v.ur - v.ul) pp[l] += w[l] * val;
+                __builtin_amdgcn_wave_barrier();
+              }
+            } else {
+#pragma unroll
+              for (int l = 0; l < W; ++l)
+                if (l < v.ur - v.ul) atomicAdd(pp + l, w[l] * val);
+            }
+          } else if (act) {
+            T dummy = T(0);
+            slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
+          }
+          m_lo = lo - ul;
+          m_vl = c.vl - vl;
+          m_npv = npv;
+          m_span = span;
+#pragma unroll
+          for (int m = 0; m < W; ++m)
+            if (lane == m) cvs[wave][m] = cv[m];
+        } else if (act) {
+          T dummy = T(0);
+          slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
+        }
+      }
+      if (lane == 0) {
+        meta[wave][0] = m_lo;
+        meta[wave][1] = m_vl;
+        meta[wave][2] = m_npv;
+        meta[wave][3] = m_span;
+      }
+      __syncthreads();
+      // --- the four outer products, threads own distinct u ---
+      if (in_lds) {
+        for (int q = 0; q < kWaves; ++q) {
+          const int span = meta[q][3];
+          if (span == 0) continue;
+          const int lo = meta[q][0], r0 = meta[q][1], npv = meta[q][2];
+          const T *pq = profs + q * pmax;
+          for (int u = threadIdx.x; u < span; u += kThreads) {
+            const T pu = pq[u];
+            if (pu != T(0)) {
+              T *dst = win + r0 * RW + lo + u;
+#pragma unroll
+              for (int m = 0; m < W; ++m)
+                if (m < npv) dst[m * RW] += cvs[q][m] * pu;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // the window goes to this (view, tile) scratch slot; fwd_reduce sums the
+    // slots covering each detector pixel
+    if (in_lds) {
+      T *dst = scratch + slot * wcap;
+      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) dst[idx] = win[idx];
+    }
+    if (threadIdx.x == 0) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
+    __syncthreads();
+  }
+}
+
+// vol[t][k][j][i] -> volk[t][j][i][k] through a 32 (i) x 64 (k) LDS tile,
+// both sides coalesced.  grid: (ceil(nx/32) * ny, ceil(nz/64), nt).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void to_k_fastest(const T *__restrict__ vol,
+                                                         T *__restrict__ volk, int nx, int ny,
+                                                         int nz) {
+  __shared__ T tile[64][33];
+  const int nix = (nx + 31) / 32;
+  const int i0 = (blockIdx.x % nix) * 32, j = blockIdx.x / nix, k0 = blockIdx.y * 64;
+  const long phase = long(blockIdx.z) * nx * ny * nz;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int kk = ty; kk < 64; kk += 8) {
+    const int k = k0 + kk, i = i0 + tx;
+    tile[kk][tx] = (k < nz && i < nx) ? vol[phase + (long(k) * ny + j) * nx + i] : T(0);
+  }
+  __syncthreads();
+  const int lk = threadIdx.x & 63, li = threadIdx.x >> 6;  // 64 x 4
+  for (int ii = li; ii < 32; ii += 4) {
+    const int k = k0 + lk, i = i0 + ii;
+    if (k < nz && i < nx) volk[phase + (long(j) * nx + i) * nz + k] = tile[lk][ii];
+  }
+}
+
+// Sum of the scratch windows over the tiles, per detector pixel:
+// sino[view][v][u] += sum_t window_t(u, v).  Pixel tile 64 (u) x 16 (v).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__restrict__ scratch,
+                                                       const int4 *__restrict__ hdr,
+                                                       T *__restrict__ sino, int view_base,
+                                                       int ntiles, int cap) {
+  __shared__ int hits[kThreads];
+  __shared__ int nhit;
+  const int npu = (P.nu + 63) / 64;
+  const int pu = blockIdx.x % npu, pv = blockIdx.x / npu;
+  const int vloc = blockIdx.y;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int u = pu * 64 + tx, v0 = pv * 16;
+  const int u_lo = pu * 64, u_hi = u_lo + 64;
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+  const int4 *h = hdr + long(vloc) * ntiles;
+  for (int t0 = 0; t0 < ntiles; t0 += kThreads) {
+    // each thread tests one header; the overlapping ones are compacted in LDS
+    if (threadIdx.x == 0) nhit = 0;
+    __syncthreads();
+    const int t = t0 + threadIdx.x;
+    if (t < ntiles) {
+      const int4 r = h[t];
+      if (r.z != 0 && r.x < u_hi && r.x + r.z > u_lo && r.y < v0 + 16 && r.y + r.w > v0)
+        hits[atomicAdd(&nhit, 1)] = t;
+    }
+    __syncthreads();
+    const int n = nhit;
+    for (int e = 0; e < n; ++e) {
+      const int tt = hits[e];
+      const int4 r = h[tt];
+      const T *src = scratch + (long(vloc) * ntiles + tt) * cap;
+      const bool uin = u >= r.x && u < r.x + r.z;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int v = v0 + ty + 4 * q;
+        if (uin && v >= r.y && v < r.y + r.w) acc[q] += src[(v - r.y) * r.z + (u - r.x)];
+      }
+    }
+    __syncthreads();
+  }
+  T *out = sino + long(view_base + vloc) * P.nu * P.nv;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int v = v0 + ty + 4 * q;
+    if (u < P.nu && v < P.nv) out[long(v) * P.nu + u] += acc[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward
+// ---------------------------------------------------------------------------
+template <bool CONE, int W, int CX, int CY, typename T>
+__global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__ sino,
+                                                T *__restrict__ vol,
+                                                const int *__restrict__ seg_begin, int atomic_out,
+                                                int lds_cap, int pmax) {
+  constexpr int NCOL = CX * CY / kWaves;
+  static_assert(NCOL <= 64, "one column per lane");
+  T *region = reinterpret_cast<T *>(g_smem);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T *qrow = region + lds_cap + wave * pmax;
+  __shared__ int red[4 * kWaves];
+  TileMap tm;
+  tm.init(P, CX, CY);
+  const int k = tm.bz * kKT + lane;
+  const bool kin = k < P.nz;
+  const int kf = tm.bz * kKT, kl = min(kf + kKT - 1, P.nz - 1);
+  const long npix = long(P.nu) * P.nv;
+  const int s_begin = seg_begin[blockIdx.y], s_end = seg_begin[blockIdx.y + 1];
+
+  for (int s = s_begin; s < s_end; ++s) {
+    const Segment seg = P.segs[s];
+    T acc0[NCOL], acc1[NCOL];
+#pragma unroll
+    for (int cc = 0; cc < NCOL; ++cc) acc0[cc] = acc1[cc] = T(0);
+
+    for (int n = 0; n < seg.count; ++n) {
+      const int view = P.order[seg.start + n];
+      const ViewConst &g = P.views[view];
+      const T *in = sino + long(view) * npix;
+      ColLane<T> cl;
+      column_table<CONE, CX, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
+      if (cl.bad) atomicOr(P.err, 1);
+      int ul, ur, vl, vr;
+      block_window(cl, red, ul, ur, vl, vr);
+      const bool any = ur > ul && vr > vl;
+      const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
+      const bool in_lds = any && RW * RH <= lds_cap;
+      if (in_lds) {
+        for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) {
+          const int r = idx / RW, cI = idx - r * RW;
+          region[idx] = in[long(vl + r) * P.nu + (ul + cI)];
+        }
+      }
+      __syncthreads();
+      if (any) {
+        const unsigned long long live_mask = __ballot(cl.live);
+#pragma unroll 1
+        for (int cc = 0; cc < NCOL; ++cc) {
+          if (!((live_mask >> cc) & 1ull)) continue;
+          Col<T> c;
+          int lo, hi, i, j;
+          bcast_col(cl, cc, c, lo, hi, i, j);
+          const int span = hi - lo;
+          const int npv = c.vr - c.vl;
+          const bool use_q = in_lds && span <= pmax && npv <= W;
+          T cv[W];
+          vweights<W>(c, lane, cv);
+          T contrib = T(0);
+          if (use_q) {
+            const T *src0 = region + (c.vl - vl) * RW + (lo - ul);
+            for (int u = lane; u < span; u += 64) {
+              const T *src = src0 + u;
+              T qv = T(0);
+#pragma unroll
+              for (int m = 0; m < W; ++m)
+                if (m < npv) qv += cv[m] * src[m * RW];
+              qrow[u] = qv;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (kin) {
+              Vox<T> v;
+              voxel(P, g, c, k, v);
+              if (v.ul < v.ur) {
+                if (v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W) {
+                  T w[W];
+                  uweights<W>(v, w);
+                  const T *qq = qrow + (v.ul - lo);
+                  T sum = T(0);
+#pragma unroll
+                  for (int l = 0; l < W; ++l)
+                    if (l < v.ur - v.ul) sum += w[l] * qq[l];
+                  contrib = sum * v.rho;
+                } else {
+                  slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
+                }
+              }
+            }
+            __builtin_amdgcn_wave_barrier();
+          } else if (kin) {
+            Vox<T> v;
+            voxel(P, g, c, k, v);
+            if (v.ul < v.ur) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
+          }
+          if (seg.t0 == seg.t1) {
+            acc0[cc] += T(g.w0 + g.w1) * contrib;
+          } else {
+            acc0[cc] += T(g.w0) * contrib;
+            acc1[cc] += T(g.w1) * contrib;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (kin) {
+#pragma unroll
+      for (int cc = 0; cc < NCOL; ++cc) {
+        const int col = wave * NCOL + cc;
+        const int i = tm.bx * CX + col % CX, j = tm.by * CY + col / CX;
+        if (i >= P.nx || j >= P.ny) continue;
+        const long idx = (long(k) * P.ny + j) * P.nx + i;
+        T *d0 = vol + long(seg.t0) * P.vox_per_phase + idx;
+        if (atomic_out) {
+          if (acc0[cc] != T(0)) unsafeAtomicAdd(d0, acc0[cc]);
+        } else {
+          *d0 += acc0[cc];
+        }
+        if (seg.t0 != seg.t1) {
+          T *d1 = vol + long(seg.t1) * P.vox_per_phase + idx;
+          if (atomic_out) {
+            if (acc1[cc] != T(0)) unsafeAtomicAdd(d1, acc1[cc]);
+          } else {
+            *d1 += acc1[cc];
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace axial
+
+// Forward tile: 16x16 columns x 64 k; backward tile: 8x8 columns x 64 k.
+// Forward tile: 16x16 columns x 64 k; backward tile: 8x8 columns x 64 k;
+// one shared 32 KiB detector window per workgroup.
+constexpr int kFwdCX = 16, kFwdCY = 16, kBwdCX = 8, kBwdCY = 8;
+constexpr int kAxialRegionBytes = 32768;
+constexpr int kAxialPmax = 256;  // longest column profile held in LDS
+
+
+int axial_tiles(const DevPlan &P, int job) {
+  const int cx = job == 0 ? kFwdCX : kBwdCX, cy = job == 0 ? kFwdCY : kBwdCY;
+  return ((P.nx + cx - 1) / cx) * ((P.ny + cy - 1) / cy) * ((P.nz + axial::kKT - 1) / axial::kKT);
+}
+
+// Scratch of the forward reduction: one detector window per (view, tile)
+// for `chunk` views, plus one header per slot.
+size_t axial_scratch_bytes(const DevPlan &P, int chunk) {
+  const size_t slots = size_t(chunk) * size_t(axial_tiles(P, 0));
+  return slots * size_t(kAxialRegionBytes) + slots * sizeof(int4);
+}
+
+template <bool CONE, int W, typename T>
+hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
+                          const int *seg_begin, int nrows, int views_per_block, int atomic_out,
+                          int pmax, void *scratch, void *hdr, int chunk, void *volk_buf,
+                          int plain, hipStream_t st) {
+  if (job == 0) {
+    const int wcap = kAxialRegionBytes / int(sizeof(T));
+    const int lds_bytes = (wcap + axial::kWaves * pmax) * int(sizeof(T));
+    const unsigned ntiles = unsigned(axial_tiles(P, 0));
+    const int npt = ((P.nu + 63) / 64) * ((P.nv + 15) / 16);
+    T *volk = static_cast<T *>(volk_buf);
+    hipLaunchKernelGGL(axial::to_k_fastest<T>, dim3(((P.nx + 31) / 32) * P.ny, (P.nz + 63) / 64, P.nt),
+                       dim3(axial::kThreads), 0, st, static_cast<const T *>(in), volk, P.nx, P.ny,
+                       P.nz);
+    for (int vb = 0; vb < P.nviews; vb += chunk) {
+      const int ve = vb + chunk < P.nviews ? vb + chunk : P.nviews;
+      dim3 grid(ntiles, (ve - vb + views_per_block - 1) / views_per_block);
+      hipLaunchKernelGGL((axial::fwd<CONE, W, kFwdCX, kFwdCY, T>), grid, dim3(axial::kThreads),
+                         lds_bytes, st, P, static_cast<const T *>(volk), static_cast<T *>(out),
+                         views_per_block, wcap, pmax, vb, ve, static_cast<T *>(scratch),
+                         static_cast<int4 *>(hdr), plain);
+      hipLaunchKernelGGL(axial::fwd_reduce<T>, dim3(npt, ve - vb), dim3(axial::kThreads), 0, st,
+                         P, static_cast<const T *>(scratch), static_cast<const int4 *>(hdr),
+                         static_cast<T *>(out), vb, int(ntiles), wcap);
+    }
+  } else {
+    const int lds_cap = kAxialRegionBytes / int(sizeof(T));
+    const int lds_bytes = kAxialRegionBytes + axial::kWaves * pmax * int(sizeof(T));
+    dim3 grid(unsigned(axial_tiles(P, 1)), nrows);
+    hipLaunchKernelGGL((axial::bwd<CONE, W, kBwdCX, kBwdCY, T>), grid, dim3(axial::kThreads),
+                       lds_bytes, st, P, static_cast<const T *>(in), static_cast<T *>(out),
+                       seg_begin, atomic_out, lds_cap, pmax);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
+                        void *out, const int *seg_begin, int nrows, int views_per_block,
+                        int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
+                        void *volk, int plain, hipStream_t st) {
+#define AX_CASE(C, WW, TT)                                                                   \
+  if (cone == C && w == WW && f64 == (sizeof(TT) == 8))                                     \
+    return launch_axial_t<C, WW, TT>(P, job, in, out, seg_begin, nrows, views_per_block,   \
+                                     atomic_out, pmax, scratch, hdr, chunk, volk, plain, st);
+  AX_CASE(true, 9, float)
+  AX_CASE(true, 13, float)
+  AX_CASE(false, 9, float)
+  AX_CASE(false, 13, float)
+#undef AX_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace trtt

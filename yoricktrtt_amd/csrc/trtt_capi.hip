@@ -17,6 +17,12 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
                             int views_per_block, int atomic_out, hipStream_t st);
 hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long n, double *fx,
                            hipStream_t st);
+hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
+                        void *out, const int *seg_begin, int nrows, int views_per_block,
+                        int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
+                        void *volk, int plain, hipStream_t st);
+int axial_tiles(const DevPlan &P, int job);
+size_t axial_scratch_bytes(const DevPlan &P, int chunk);
 }  // namespace trtt
 
 using trtt::ViewConst;
@@ -152,6 +158,67 @@ void view_consts_lf(ViewConst &g, const lf2d_parameters_t *p, double beta, doubl
 
 #pragma clang fp contract(on)
 
+// Fast axial path eligibility: closed-form FP32 3-D plans whose views all
+// rotate about the object z axis (o3 = den_z = num_v_z = 0), with the whole
+// volume in front of the source.  Returns the footprint width bound W
+// (9 or 13 pixels) and the column profile length, or W = 0.
+void axial_analysis(trtt_hip_plan *p) {
+  p->axial_w = 0;
+  p->axial_pmax = 0;
+  if (p->table || p->precision != TRTT_HIP_F32) return;
+  if (p->kind != trtt::CB3D && p->kind != trtt::PB3D) return;
+  const bool cone = p->kind == trtt::CB3D;
+  const double s = p->s_scl, hs0 = 0.5 * (p->s_deg + 1);
+  const double hu_init = s / p->u_scl, hv_init = s / p->v_scl;
+  double hu_max = 0.0, hv_max = 0.0, duc_max = 0.0, duc_min = 1e300, dhs_rate = 0.0;
+  const double ci[2] = {0.0, double(p->nx - 1)}, cj[2] = {0.0, double(p->ny - 1)};
+  for (const ViewConst &g : p->h_views) {
+    if (g.nvz != 0.0) return;
+    if (cone) {
+      if (g.o3 != 0.0 || g.dz != 0.0) return;
+      double wmin = 1e300, wmax = -1e300, qmax = 0.0, qmin = 1e300;
+      for (double i : ci)
+        for (double j : cj) {
+          const double wk = (g.o4 + g.o2 * j) + g.o1 * i;
+          const double den = (g.dc + g.dy * j) + g.dx * i;
+          wmin = std::min(wmin, wk);
+          wmax = std::max(wmax, wk);
+          if (!(den * g.a4m > 0.0)) return;
+          qmax = std::max(qmax, std::fabs(g.a4m / den));
+          qmin = std::min(qmin, std::fabs(g.a4m / den));
+        }
+      if (!(wmin > 0.0) && !(wmax < 0.0)) return;  // source plane crosses the volume
+      const double magn = std::fabs(g.a4m) / std::min(std::fabs(wmin), std::fabs(wmax));
+      const double u_ext = (std::max(std::fabs(g.a12), std::fabs(p->nu - g.a12)) +
+                            8.0 * hu_init * magn + 4.0) * p->u_scl;
+      const double v_ext = (std::max(std::fabs(g.a8), std::fabs(p->nv - g.a8)) +
+                            8.0 * hv_init * magn + 4.0) * p->v_scl;
+      const double du = std::sqrt(1.0 + u_ext * u_ext / g.a4sqr);
+      const double dv = std::sqrt(1.0 + v_ext * v_ext / g.a4sqr);
+      hu_max = std::max(hu_max, hu_init * magn * du);
+      hv_max = std::max(hv_max, hv_init * magn * dv);
+      duc_max = std::max(duc_max, qmax * std::fabs(g.nuz));
+      duc_min = std::min(duc_min, qmin * std::fabs(g.nuz));
+      // |d hs / d k| <= hs0 hu du' duc with du' <= |u| u_scl / a4^2
+      dhs_rate = std::max(dhs_rate, hs0 * hu_init * magn * u_ext * p->u_scl / g.a4sqr *
+                                        qmax * std::fabs(g.nuz));
+    } else {
+      hu_max = std::max(hu_max, g.pb_hu);
+      hv_max = std::max(hv_max, g.pb_hv);
+      duc_max = std::max(duc_max, std::fabs(g.nuz));
+      duc_min = std::min(duc_min, std::fabs(g.nuz));
+    }
+  }
+  const double npix = std::floor(2.0 * hs0 * std::max(hu_max, hv_max)) + 2.0;
+  const int w = npix <= 9.0 ? 9 : (npix <= 13.0 ? 13 : 0);
+  const double pmax = std::ceil(63.0 * duc_max + 2.0 * hs0 * hu_max) + 8.0;
+  if (w == 0 || !(pmax <= 256.0)) return;
+  p->axial_w = w;
+  p->axial_pmax = int(pmax);
+  // consecutive voxels of a column start their windows on distinct pixels
+  p->axial_plain = duc_min - 2.0 * dhs_rate > 1.01;
+}
+
 // Number of workgroups a launch should offer to fill 256 CUs several times.
 constexpr long kTargetBlocks = 4096;
 
@@ -264,6 +331,9 @@ void release(trtt_hip_plan *p) {
   hipFree(p->d_seg_begin);
   hipFree(p->d_table);
   hipFree(p->d_err);
+  hipFree(p->d_scratch);
+  hipFree(p->d_hdr);
+  hipFree(p->d_volk);
   delete p;
 }
 
@@ -306,6 +376,9 @@ unsigned grid_for(long n) {
   return unsigned(std::max(1L, std::min(b, 8192L)));
 }
 
+hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hipStream_t st);
+bool alloc_axial_scratch(trtt_hip_plan *p);
+
 // Run one view of a plan on host FP64 arrays: out_host += A in_host (job 0)
 // or A' in_host (job != 0).
 int run_dropin(trtt_hip_plan *p, double *out, const double *in, int job) {
@@ -338,12 +411,7 @@ int run_dropin(trtt_hip_plan *p, double *out, const double *in, int job) {
   }
   if (!hip_ok(hipMemsetAsync(d_out, 0, esz * nout, st), "hipMemset")) return TRTT_FAILURE;
   if (!hip_ok(hipMemsetAsync(p->d_err, 0, sizeof(int), st), "hipMemset")) return TRTT_FAILURE;
-  const int nrows = int(p->h_seg_begin.size()) - 1;
-  if (!hip_ok(trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, job, d_in,
-                                     d_out, p->d_seg_begin, nrows, views_per_block(p),
-                                     nrows > 1 ? 1 : 0, st),
-              "kernel launch"))
-    return TRTT_FAILURE;
+  if (!hip_ok(launch_plan(p, job, d_in, d_out, st), "kernel launch")) return TRTT_FAILURE;
   std::vector<double> res(nout);
   if (f64) {
     if (!hip_ok(hipMemcpy(res.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost),
@@ -435,7 +503,50 @@ trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, lon
     release(p);
     return nullptr;
   }
+  axial_analysis(p);
+  if (p->axial_w > 0 && !alloc_axial_scratch(p)) {
+    release(p);
+    return nullptr;
+  }
   return p;
+}
+
+// Scratch of the axial forward: windows of `chunk` views at a time, at most
+// 1 GiB (views are processed in chunks that fit).
+bool alloc_axial_scratch(trtt_hip_plan *p) {
+  const size_t per_view = trtt::axial_scratch_bytes(p->dev, 1);
+  const size_t budget = size_t(1) << 30;
+  long chunk = long(budget / per_view);
+  chunk = std::max(1L, std::min(chunk, p->nviews));
+  p->axial_chunk = int(chunk);
+  const size_t slots = size_t(chunk) * size_t(trtt::axial_tiles(p->dev, 0));
+  const size_t win_bytes = per_view * size_t(chunk) - slots * sizeof(int4);
+  if (!hip_ok(hipMalloc(&p->d_scratch, win_bytes), "hipMalloc scratch")) return false;
+  if (!hip_ok(hipMalloc(&p->d_hdr, slots * sizeof(int4)), "hipMalloc scratch")) return false;
+  // k-fastest copy of the volume series read by the forward
+  const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
+  if (!hip_ok(hipMalloc(&p->d_volk, vbytes), "hipMalloc scratch")) return false;
+  return true;
+}
+
+// One launch of a plan (forward job 0 / backward job 1).
+hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hipStream_t st) {
+  const int nrows = int(p->h_seg_begin.size()) - 1;
+  const bool f64 = p->precision == TRTT_HIP_F64;
+  if (p->axial_w > 0) {
+    const long ntiles = trtt::axial_tiles(p->dev, job);
+    long vpb = (ntiles * p->nviews) / kTargetBlocks;
+    vpb = std::max(1L, std::min(vpb, 8L));
+    return trtt::launch_axial(p->kind == trtt::CB3D, p->axial_w, f64, p->dev, job, in, out,
+                              p->d_seg_begin, nrows, int(vpb), nrows > 1 ? 1 : 0, p->axial_pmax,
+                              p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
+                              p->axial_plain ? 1 : 0, st);
+  }
+  if (job == 0)
+    return trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, 0, in, out,
+                                  p->d_seg_begin, nrows, views_per_block(p), 0, st);
+  return trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, 1, in, out,
+                                p->d_seg_begin, nrows, int(p->nviews), nrows > 1 ? 1 : 0, st);
 }
 
 }  // namespace
@@ -512,25 +623,14 @@ trtt_hip_plan_t *trtt_hip_plan_create_lf2d(int precision, const lf2d_parameters_
 
 int trtt_hip_forward(trtt_hip_plan_t *p, const void *vol, void *sino, void *stream) {
   if (!p || !vol || !sino) return fail("null argument");
-  const int nrows = int(p->h_seg_begin.size()) - 1;
-  return hip_ok(trtt::launch_projector(p->kind, p->table, p->quirk_last_bin,
-                                       p->precision == TRTT_HIP_F64, p->dev, 0, vol, sino,
-                                       p->d_seg_begin, nrows, views_per_block(p), 0,
-                                       static_cast<hipStream_t>(stream)),
-                "forward launch")
+  return hip_ok(launch_plan(p, 0, vol, sino, static_cast<hipStream_t>(stream)), "forward launch")
              ? TRTT_SUCCESS
              : TRTT_FAILURE;
 }
 
 int trtt_hip_backward(trtt_hip_plan_t *p, const void *sino, void *vol, void *stream) {
   if (!p || !vol || !sino) return fail("null argument");
-  const int nrows = int(p->h_seg_begin.size()) - 1;
-  int vpb = int(p->nviews);
-  if (p->kind == trtt::LF2D) vpb = int(p->nviews);
-  return hip_ok(trtt::launch_projector(p->kind, p->table, p->quirk_last_bin,
-                                       p->precision == TRTT_HIP_F64, p->dev, 1, sino, vol,
-                                       p->d_seg_begin, nrows, vpb, nrows > 1 ? 1 : 0,
-                                       static_cast<hipStream_t>(stream)),
+  return hip_ok(launch_plan(p, 1, sino, vol, static_cast<hipStream_t>(stream)),
                 "backward launch")
              ? TRTT_SUCCESS
              : TRTT_FAILURE;
@@ -553,6 +653,8 @@ int trtt_hip_plan_error_flag(trtt_hip_plan_t *p) {
   if (hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
   return err;
 }
+
+int trtt_hip_plan_path(const trtt_hip_plan_t *p) { return p ? p->axial_w : -1; }
 
 long trtt_hip_plan_nviews(const trtt_hip_plan_t *p) { return p ? p->nviews : 0; }
 long trtt_hip_plan_volume_size(const trtt_hip_plan_t *p) {

@@ -86,4 +86,13 @@ struct trtt_hip_plan {
   long m = 0;
   double step = 0.0, coord0 = 0.0;
   trtt::DevPlan dev{};
+  // fast axial path (closed-form cubic, FP32, rotation axis = z = u):
+  // footprint width bound W (0: not eligible) and column profile length
+  int axial_w = 0;
+  int axial_pmax = 0;
+  bool axial_plain = false;   // profile accumulation without LDS atomics
+  int axial_chunk = 1;      // views per forward scratch round
+  void *d_scratch = nullptr;  // forward windows, one per (view, tile)
+  void *d_hdr = nullptr;      // int4 (ul, vl, width, height) per window
+  void *d_volk = nullptr;     // volume series with k fastest (forward input)
 };

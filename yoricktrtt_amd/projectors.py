@@ -292,6 +292,11 @@ class Projector:
                                                self._stream(stream)) != 0:
             raise RuntimeError(_lib.last_error())
 
+    @property
+    def path(self) -> int:
+        """0: general kernels; W > 0: axial closed-form kernels (width bound W)."""
+        return self.lib.trtt_hip_plan_path(self.handle)
+
     def error_flag(self) -> int:
         return self.lib.trtt_hip_plan_error_flag(self.handle)
 
