@@ -274,12 +274,12 @@ struct ColLane {
   bool bad;     // wk == 0 (voxel in the source plane)
 };
 
-template <bool CONE, int CX, int NCOL, typename T>
+template <bool CONE, int CX, int CY, int NCOL, typename T>
 __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &g, const TileMap &tm,
                                              int wave, int lane, int kf, int kl, ColLane<T> &cl) {
   const int col = wave * NCOL + lane;
   cl.i = tm.bx * CX + col % CX;
-  cl.j = tm.by * (NCOL * kWaves / CX) + col / CX;
+  cl.j = tm.by * CY + col / CX;
   cl.live = false;
   cl.bad = false;
   cl.lo = cl.hi = 0;
@@ -337,28 +337,26 @@ __device__ __forceinline__ void block_window(const ColLane<T> &cl, int *red, int
 }
 
 // ---------------------------------------------------------------------------
-// Forward.  Rounds of four columns (one per wave): each wave accumulates its
-// column's axial profile in wave-private LDS (plain read-modify-write when
-// the host has shown that the lanes of a wave hit distinct pixels at each
-// step, LDS atomics otherwise); after a barrier the 256 threads apply the
-// four outer products P x cv to the workgroup's detector window with each
-// thread owning distinct u, so the window needs no atomics either.
+// Forward.  One wave per workgroup: the wave owns a tile of CX x CY columns
+// x 64 k and a private detector window in LDS, so no barrier or atomic is
+// needed between waves.  Per column the lanes accumulate the axial profile
+// P[u] = sum_k x_k rho_k w_k(u) in LDS (plain read-modify-write when the
+// host has shown that the lanes hit distinct pixels at each step, LDS atomics
+// otherwise), then add the outer product P x cv to the window with the lanes
+// on distinct window columns.  The window goes to a scratch slot; fwd_reduce
+// sums the slots covering each detector pixel.
 // ---------------------------------------------------------------------------
 template <bool CONE, int W, int CX, int CY, typename T>
-__global__ __launch_bounds__(kThreads) void fwd(DevPlan P, const T *__restrict__ vol,
-                                                T *__restrict__ sino, int views_per_block,
-                                                int wcap, int pmax, int view_base,
-                                                int view_end, T *__restrict__ scratch,
-                                                int4 *__restrict__ hdr, int plain) {
-  constexpr int NCOL = CX * CY / kWaves;  // columns per wave
+__global__ __launch_bounds__(64) void fwd(DevPlan P, const T *__restrict__ vol,
+                                          T *__restrict__ sino, int views_per_block, int wcap,
+                                          int pmax, int view_base, int view_end,
+                                          T *__restrict__ scratch, int4 *__restrict__ hdr,
+                                          int plain) {
+  constexpr int NCOL = CX * CY;  // columns of the wave
   static_assert(NCOL <= 64, "one column per lane");
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x;
   T *win = reinterpret_cast<T *>(g_smem);
-  T *profs = win + wcap;
-  T *prof = profs + wave * pmax;
-  __shared__ int red[4 * kWaves];
-  __shared__ int meta[kWaves][4];   // per wave: lo, vl, npv, span (span 0: nothing)
-  __shared__ T cvs[kWaves][W];
+  T *prof = win + wcap;
   TileMap tm;
   tm.init(P, CX, CY);
   const int k = tm.bz * kKT + lane;
@@ -375,21 +373,23 @@ __global__ __launch_bounds__(kThreads) void fwd(DevPlan P, const T *__restrict__
     const long slot = long(view - view_base) * ntiles + blockIdx.x;
 
     ColLane<T> cl;
-    column_table<CONE, CX, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
+    column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
     if (cl.bad) atomicOr(P.err, 1);
-    int ul, ur, vl, vr;
-    block_window(cl, red, ul, ur, vl, vr);
+    const int ul = wred_min(cl.live ? cl.lo : INT_MAX);
+    const int ur = wred_max(cl.live ? cl.hi : INT_MIN);
+    const int vl = wred_min(cl.live ? cl.c.vl : INT_MAX);
+    const int vr = wred_max(cl.live ? cl.c.vr : INT_MIN);
     const bool any = ur > ul && vr > vl;
     const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && RW * RH <= wcap;
     if (in_lds)
-      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) win[idx] = T(0);
-    const unsigned long long live_mask = __ballot(cl.live);
+      for (int idx = lane; idx < RW * RH; idx += 64) win[idx] = T(0);
+    __builtin_amdgcn_wave_barrier();
 
-    for (int cc = 0; cc < NCOL; ++cc) {
-      // --- this wave's column cc: profile in LDS ---
-      int m_lo = 0, m_vl = 0, m_npv = 0, m_span = 0;
-      if (any && ((live_mask >> cc) & 1ull)) {
+    if (any) {
+      const unsigned long long live_mask = __ballot(cl.live);
+      for (int cc = 0; cc < NCOL; ++cc) {
+        if (!((live_mask >> cc) & 1ull)) continue;
         Col<T> c;
         int lo, hi, i, j;
         bcast_col(cl, cc, c, lo, hi, i, j);
@@ -413,18 +413,24 @@ __global__ __launch_bounds__(kThreads) void fwd(DevPlan P, const T *__restrict__
         if (use_prof) {
           for (int u = lane; u < span; u += 64) prof[u] = T(0);
           __builtin_amdgcn_wave_barrier();
-          const bool fast = act && !v.clipped && v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W;
+          const bool fast = act && v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W;
           if (fast) {
             T w[W];
             uweights<W>(v, w);
             T *pp = prof + (v.ul - lo);
-            if (plain) {
+            // unclipped windows start on strictly monotone pixels: plain
+            // accumulation; windows clamped at a detector edge may share
+            // their first pixel: LDS atomics, issued after the plain steps
+            const bool pl = plain && !v.clipped;
+            if (pl) {
 #pragma unroll
               for (int l = 0; l < W; ++l) {
                 if (l < v.ur - v.ul) pp[l] += w[l] * val;
                 __builtin_amdgcn_wave_barrier();
               }
-            } else {
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (!pl) {
 #pragma unroll
               for (int l = 0; l < W; ++l)
                 if (l < v.ur - v.ul) atomicAdd(pp + l, w[l] * val);
@@ -433,53 +439,34 @@ __global__ __launch_bounds__(kThreads) void fwd(DevPlan P, const T *__restrict__
             T dummy = T(0);
             slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
           }
-          m_lo = lo - ul;
-          m_vl = c.vl - vl;
-          m_npv = npv;
-          m_span = span;
+          __builtin_amdgcn_wave_barrier();
+          // outer product P x cv; the lanes sit on distinct window columns and
+          // the rows of one column are distinct: one batch of reads, FMAs and
+          // writes per lane
+          T *dst0 = win + (c.vl - vl) * RW + (lo - ul);
+          for (int u = lane; u < span; u += 64) {
+            const T pu = prof[u];
+            T *dst = dst0 + u;
+            T t[W];
 #pragma unroll
-          for (int m = 0; m < W; ++m)
-            if (lane == m) cvs[wave][m] = cv[m];
+            for (int m = 0; m < W; ++m) t[m] = m < npv ? dst[m * RW] : T(0);
+#pragma unroll
+            for (int m = 0; m < W; ++m)
+              if (m < npv) dst[m * RW] = t[m] + cv[m] * pu;
+          }
+          __builtin_amdgcn_wave_barrier();
         } else if (act) {
           T dummy = T(0);
           slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
         }
       }
-      if (lane == 0) {
-        meta[wave][0] = m_lo;
-        meta[wave][1] = m_vl;
-        meta[wave][2] = m_npv;
-        meta[wave][3] = m_span;
-      }
-      __syncthreads();
-      // --- the four outer products, threads own distinct u ---
-      if (in_lds) {
-        for (int q = 0; q < kWaves; ++q) {
-          const int span = meta[q][3];
-          if (span == 0) continue;
-          const int lo = meta[q][0], r0 = meta[q][1], npv = meta[q][2];
-          const T *pq = profs + q * pmax;
-          for (int u = threadIdx.x; u < span; u += kThreads) {
-            const T pu = pq[u];
-            if (pu != T(0)) {
-              T *dst = win + r0 * RW + lo + u;
-#pragma unroll
-              for (int m = 0; m < W; ++m)
-                if (m < npv) dst[m * RW] += cvs[q][m] * pu;
-            }
-          }
-        }
-      }
-      __syncthreads();
     }
-    // the window goes to this (view, tile) scratch slot; fwd_reduce sums the
-    // slots covering each detector pixel
     if (in_lds) {
       T *dst = scratch + slot * wcap;
-      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) dst[idx] = win[idx];
+      for (int idx = lane; idx < RW * RH; idx += 64) dst[idx] = win[idx];
     }
-    if (threadIdx.x == 0) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
-    __syncthreads();
+    if (lane == 0) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -589,7 +576,7 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
       const ViewConst &g = P.views[view];
       const T *in = sino + long(view) * npix;
       ColLane<T> cl;
-      column_table<CONE, CX, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
+      column_table<CONE, CX, CY, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
       if (cl.bad) atomicOr(P.err, 1);
       int ul, ur, vl, vr;
       block_window(cl, red, ul, ur, vl, vr);
@@ -691,10 +678,11 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
 }  // namespace axial
 
 // Forward tile: 16x16 columns x 64 k; backward tile: 8x8 columns x 64 k.
-// Forward tile: 16x16 columns x 64 k; backward tile: 8x8 columns x 64 k;
-// one shared 32 KiB detector window per workgroup.
-constexpr int kFwdCX = 16, kFwdCY = 16, kBwdCX = 8, kBwdCY = 8;
+// Forward: one wave per workgroup, tile 8x4 columns x 64 k, private 16 KiB
+// window.  Backward: four waves, tile 8x8 columns x 64 k, shared 32 KiB window.
+constexpr int kFwdCX = 8, kFwdCY = 4, kBwdCX = 8, kBwdCY = 8;
 constexpr int kAxialRegionBytes = 32768;
+constexpr int kFwdWindowBytes = 16384;
 constexpr int kAxialPmax = 256;  // longest column profile held in LDS
 
 
@@ -707,7 +695,7 @@ int axial_tiles(const DevPlan &P, int job) {
 // for `chunk` views, plus one header per slot.
 size_t axial_scratch_bytes(const DevPlan &P, int chunk) {
   const size_t slots = size_t(chunk) * size_t(axial_tiles(P, 0));
-  return slots * size_t(kAxialRegionBytes) + slots * sizeof(int4);
+  return slots * size_t(kFwdWindowBytes) + slots * sizeof(int4);
 }
 
 template <bool CONE, int W, typename T>
@@ -716,8 +704,8 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
                           int pmax, void *scratch, void *hdr, int chunk, void *volk_buf,
                           int plain, hipStream_t st) {
   if (job == 0) {
-    const int wcap = kAxialRegionBytes / int(sizeof(T));
-    const int lds_bytes = (wcap + axial::kWaves * pmax) * int(sizeof(T));
+    const int wcap = kFwdWindowBytes / int(sizeof(T));
+    const int lds_bytes = (wcap + pmax) * int(sizeof(T));
     const unsigned ntiles = unsigned(axial_tiles(P, 0));
     const int npt = ((P.nu + 63) / 64) * ((P.nv + 15) / 16);
     T *volk = static_cast<T *>(volk_buf);
@@ -727,8 +715,7 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     for (int vb = 0; vb < P.nviews; vb += chunk) {
       const int ve = vb + chunk < P.nviews ? vb + chunk : P.nviews;
       dim3 grid(ntiles, (ve - vb + views_per_block - 1) / views_per_block);
-      hipLaunchKernelGGL((axial::fwd<CONE, W, kFwdCX, kFwdCY, T>), grid, dim3(axial::kThreads),
-                         lds_bytes, st, P, static_cast<const T *>(volk), static_cast<T *>(out),
+      hipLaunchKernelGGL((axial::fwd<CONE, W, kFwdCX, kFwdCY, T>), grid, dim3(64), lds_bytes, st, P, static_cast<const T *>(volk), static_cast<T *>(out),
                          views_per_block, wcap, pmax, vb, ve, static_cast<T *>(scratch),
                          static_cast<int4 *>(hdr), plain);
       hipLaunchKernelGGL(axial::fwd_reduce<T>, dim3(npt, ve - vb), dim3(axial::kThreads), 0, st,

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -217,6 +218,10 @@ void axial_analysis(trtt_hip_plan *p) {
   p->axial_pmax = int(pmax);
   // consecutive voxels of a column start their windows on distinct pixels
   p->axial_plain = duc_min - 2.0 * dhs_rate > 1.01;
+  // TRTT_HIP_PROFILE_ATOMIC=1 accumulates the axial profiles with LDS
+  // atomics even where plain accumulation is safe (same result)
+  if (const char *e = std::getenv("TRTT_HIP_PROFILE_ATOMIC"))
+    if (e[0] == '1') p->axial_plain = false;
 }
 
 // Number of workgroups a launch should offer to fill 256 CUs several times.
@@ -512,10 +517,10 @@ trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, lon
 }
 
 // Scratch of the axial forward: windows of `chunk` views at a time, at most
-// 1 GiB (views are processed in chunks that fit).
+// 4 GiB (views are processed in chunks that fit).
 bool alloc_axial_scratch(trtt_hip_plan *p) {
   const size_t per_view = trtt::axial_scratch_bytes(p->dev, 1);
-  const size_t budget = size_t(1) << 30;
+  const size_t budget = size_t(4) << 30;
   long chunk = long(budget / per_view);
   chunk = std::max(1L, std::min(chunk, p->nviews));
   p->axial_chunk = int(chunk);
