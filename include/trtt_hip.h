@@ -187,6 +187,26 @@ long trtt_hip_plan_volume_size(const trtt_hip_plan_t *plan);   /* nt*nz*ny*nx */
 long trtt_hip_plan_sino_size(const trtt_hip_plan_t *plan);     /* nviews*nv*nu */
 int trtt_hip_plan_destroy(trtt_hip_plan_t *plan);
 
+/* ===================================================================== */
+/* Part 3 -- regularisation on device (stand-ins for YTotVar rgl_totvar and */
+/* SPL spl_interp_apply, Ytrtt4D_optim.i:448-482; no source in the        */
+/* reference tree)                                                         */
+/* ===================================================================== */
+
+/* Isotropic TV with threshold of a series of nt volumes [nt][nz][ny][nx]:
+ *   *f_dev += weight * sum_i (sqrt(eps^2 + |D x|_i^2) - eps),  g += its gradient,
+ * D = forward differences in x, y, z (zero at the last sample) and, for
+ * nt > 1, wt times the periodic difference in t.  `work` holds
+ * (nt > 1 ? 4 : 3) * nt*nz*ny*nx elements of the plan precision. */
+int trtt_hip_tv(const void *x, void *g, void *work, double *f_dev, long nx, long ny, long nz,
+                long nt, double weight, double wt, double eps, int precision, void *stream);
+
+/* Samples at the integer nodes of cubic B-spline coefficients, per volume:
+ * dst (+)= S src with S = [1/6, 2/3, 1/6] along x, y and z, zero padding
+ * (S is symmetric: it is also its own transpose).  src != dst. */
+int trtt_hip_bspline_filter(const void *src, void *dst, long nx, long ny, long nz, long nt,
+                            int accumulate, int precision, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,8 @@ _SIGS = {
     "trtt_hip_plan_volume_size": ([vp], L),
     "trtt_hip_plan_sino_size": ([vp], L),
     "trtt_hip_plan_destroy": ([vp], I),
+    "trtt_hip_tv": ([vp, vp, vp, vp, L, L, L, L, D, D, D, I, vp], I),
+    "trtt_hip_bspline_filter": ([vp, vp, L, L, L, L, I, I, vp], I),
     "trtt_xform3d_new": ([], vp),
     "trtt_xform3d_delete": ([vp], None),
     "trtt_xform3d_move": ([vp, D, D, D, vp], I),
