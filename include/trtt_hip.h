@@ -207,6 +207,68 @@ int trtt_hip_tv(const void *x, void *g, void *work, double *f_dev, long nx, long
 int trtt_hip_bspline_filter(const void *src, void *dst, long nx, long ny, long nz, long nt,
                             int accumulate, int precision, void *stream);
 
+/* ===================================================================== */
+/* Part 4 -- learned warm start: BF16 3-D U-Net layers on the matrix      */
+/* cores (no counterpart in the reference; used by the warm start of the  */
+/* VMLM-B reconstruction)                                                  */
+/* ===================================================================== */
+/* Tensors are NDHWC: [n][d][h][w][c].  BF16 tensors are raw 16-bit words
+ * (`void *`); FP32 master weights are w[cout][kp] with k = tap*cin + c,
+ * tap = 9 (dz+1) + 3 (dy+1) + (dx+1), zero past taps*cin; kp % 16 == 0.
+ * Every function returns 0, or -1 on a bad argument or launch error. */
+
+/* y = act(conv(x, wk) + bias) with zero padding, ks = 3 or 1 (taps = ks^3),
+ * act = ReLU when relu != 0; y (BF16) and optional y32 (FP32) get
+ * [n][d][h][w][cout].  cout <= 128, n*d <= 65535.  Run on dY with the
+ * transposed weights wt (trtt_unet_prep_weights) it is the data gradient. */
+int trtt_unet_conv_fwd(const void *x, const void *wk, const float *bias, void *y, float *y32,
+                       int n, int d, int h, int w, int cin, int cout, int ks, int kp, int relu,
+                       void *stream);
+
+/* dw[cout][kp] += sum_p dy[p][co] xcol[p][k] and db[cout] += sum_p dy[p][co]
+ * (db may be NULL); FP32 atomics. */
+int trtt_unet_conv_wgrad(const void *x, const void *dy, float *dw, float *db, int n, int d, int h,
+                         int w, int cin, int cout, int ks, int kp, void *stream);
+
+/* BF16 operands of a layer from its FP32 master w[cout][kp]: wk[cout][kp]
+ * and (if wt != NULL) the mirrored transpose wt[cin][kpt],
+ * wt[c][tap'*cout + co] = w[co][(taps-1-tap')*cin + c]. */
+int trtt_unet_prep_weights(const float *w, void *wk, void *wt, int taps, int cin, int cout, int kp,
+                           int kpt, void *stream);
+
+/* 2x2x2 max pooling of x [n][d][h][w][c] into y with the argmax map arg
+ * (forward), or dx (FP32, fine grid, zeroed by the caller) from dy and arg
+ * (backward).  d, h, w even. */
+int trtt_unet_pool(const void *x, void *y, void *arg, float *dx, const void *dy, int n, int d,
+                   int h, int w, int c, int backward, void *stream);
+
+/* x2 nearest upsampling of src [n][d/2][h/2][w/2][c] into channels
+ * [off, off + c) of dst [n][d][h][w][ctot] (forward), or its gradient: dst
+ * (coarse, c channels) = sum over the 8 children of src's channel slice. */
+int trtt_unet_upsample(const void *src, void *dst, int n, int d, int h, int w, int c, int ctot,
+                       int off, int backward, void *stream);
+
+/* dir 0: dst[p][off + ch] = src[p][ch]; dir 1: dst[p][ch] = src[p][off + ch]
+ * (src/dst with ctot channels on the sliced side). */
+int trtt_unet_channel_copy(const void *src, void *dst, long npos, int c, int ctot, int off,
+                           int dir, void *stream);
+
+/* dx = (dy32 + dy16) * (act > 0); either gradient may be NULL. */
+int trtt_unet_relu_bwd(const float *dy32, const void *dy16, const void *act, void *dx, long n,
+                       void *stream);
+
+/* b = BF16(a), round to nearest even. */
+int trtt_unet_cast_bf16(const float *a, void *b, long n, void *stream);
+
+/* out = xin + head; with target != NULL: *loss += mean (out - target)^2
+ * (FP64, loss may be NULL) and dhead (BF16, may be NULL) = 2 (out - target)/n. */
+int trtt_unet_residual_mse(const float *xin, const float *head, const float *target, float *out,
+                           void *dhead, long n, double *loss, void *stream);
+
+/* One Adam step on n FP32 parameters; lr_t = lr sqrt(1-b2^t) / (1-b1^t). */
+int trtt_unet_adam(float *p, const float *g, float *m, float *v, long n, float lr_t, float b1,
+                   float b2, float eps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

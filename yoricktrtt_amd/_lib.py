@@ -61,6 +61,16 @@ _SIGS = {
     "trtt_hip_plan_destroy": ([vp], I),
     "trtt_hip_tv": ([vp, vp, vp, vp, L, L, L, L, D, D, D, I, vp], I),
     "trtt_hip_bspline_filter": ([vp, vp, L, L, L, L, I, I, vp], I),
+    "trtt_unet_conv_fwd": ([vp, vp, vp, vp, vp, I, I, I, I, I, I, I, I, I, vp], I),
+    "trtt_unet_conv_wgrad": ([vp, vp, vp, vp, I, I, I, I, I, I, I, I, vp], I),
+    "trtt_unet_prep_weights": ([vp, vp, vp, I, I, I, I, I, vp], I),
+    "trtt_unet_pool": ([vp, vp, vp, vp, vp, I, I, I, I, I, I, vp], I),
+    "trtt_unet_upsample": ([vp, vp, I, I, I, I, I, I, I, I, vp], I),
+    "trtt_unet_channel_copy": ([vp, vp, L, I, I, I, I, vp], I),
+    "trtt_unet_relu_bwd": ([vp, vp, vp, vp, L, vp], I),
+    "trtt_unet_cast_bf16": ([vp, vp, L, vp], I),
+    "trtt_unet_residual_mse": ([vp, vp, vp, vp, vp, L, vp, vp], I),
+    "trtt_unet_adam": ([vp, vp, vp, vp, L, C.c_float, C.c_float, C.c_float, C.c_float, vp], I),
     "trtt_xform3d_new": ([], vp),
     "trtt_xform3d_delete": ([vp], None),
     "trtt_xform3d_move": ([vp, D, D, D, vp], I),
