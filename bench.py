@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--model", default="cubic", choices=["cubic", "table"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("BENCH_CPU_THREADS", 16)))
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
+    ap.add_argument("--tune-cache", default=os.path.join(HERE, "yoricktrtt_amd", "tuning",
+                                                         "tuning.json"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -127,6 +130,17 @@ def main():
     geo = geometry()
     P = Projector(geo, model=args.model, precision="f32")
     print(f"bench: plan ready ({NVIEWS} views)", file=sys.stderr, flush=True)
+    tuning = None
+    if not args.no_tune and P.path > 0:
+        from yoricktrtt_amd.autotune import autotune
+
+        tr = autotune(P, cache=args.tune_cache,
+                      log=lambda m: print("bench: tune " + m, file=sys.stderr, flush=True))
+        tuning = dict(tr["tuning"], cached=tr["cached"])
+        print(f"bench: tuning {tuning}", file=sys.stderr, flush=True)
+    timed_kernels = P.path > 0
+    if timed_kernels:
+        P.set_timing(True)
     x = torch.from_numpy(phantom()).to(dev)
     g = torch.Generator(device=dev).manual_seed(7)
     y = torch.rand((NVIEWS, NV, NU), generator=g, device=dev, dtype=torch.float32)
@@ -150,10 +164,17 @@ def main():
     torch.cuda.synchronize()
     print("bench: warmup done", file=sys.stderr, flush=True)
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    kt = []  # (forward main kernel ms, launches, backward kernel ms) per step
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(ev[k])
+        if timed_kernels:
+            # the plan's own HIP events on the launch stream, around the
+            # main forward kernel of each view chunk and the backward kernel
+            fms, nl = P.kernel_time(0)
+            bms, _ = P.kernel_time(1)
+            kt.append((fms, nl, bms))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     fwd_ms = np.mean([e[0].elapsed_time(e[1]) for e in ev])
@@ -164,9 +185,17 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     pairs_per_s = args.steps / wall
     fwd_b, bwd_b, comp_b = algorithmic_bytes()
-    dom_is_bwd = bwd_ms >= fwd_ms
-    dom_ms = bwd_ms if dom_is_bwd else fwd_ms
-    dom_bytes = bwd_b if dom_is_bwd else fwd_b
+    if kt:
+        kf = float(np.mean([t[0] for t in kt]))
+        nlaunch = kt[0][1]
+        kb = float(np.mean([t[2] for t in kt]))
+    else:
+        kf, nlaunch, kb = fwd_ms, 1, bwd_ms
+    dom_is_bwd = kb >= kf
+    # per launch: the forward runs once per chunk of views (bytes and time
+    # both split by chunk), the backward once
+    dom_ms = kb if dom_is_bwd else kf / nlaunch
+    dom_bytes = bwd_b if dom_is_bwd else fwd_b / nlaunch
     achieved = dom_bytes / (dom_ms * 1e-3)
     result = {
         "metric": "cone 256^3 fwd+back pairs/s",
@@ -186,7 +215,10 @@ def main():
                    "model": args.model},
         "roofline": {
             "bound": "hbm",
-            "kernel": "fp_backward" if dom_is_bwd else "fp_forward",
+            "kernel": ("axial::bwd" if dom_is_bwd else "axial::fwd") if kt else
+                      ("fp_backward" if dom_is_bwd else "fp_forward"),
+            "launch_ms": round(dom_ms, 3),
+            "bytes_per_launch": int(dom_bytes),
             "achieved": round(achieved / 1e9, 1),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -195,7 +227,10 @@ def main():
             "bytes_model": "per-view-operator bytes (each view reads the volume; transpose "
                            "read-modify-writes it)",
         },
-        "kernels_ms": {"fp_forward": round(fwd_ms, 3), "fp_backward": round(bwd_ms, 3)},
+        "kernels_ms": {"forward_op": round(fwd_ms, 3), "backward_op": round(bwd_ms, 3),
+                       "fwd_main": round(kf, 3), "fwd_launches": nlaunch,
+                       "bwd_main": round(kb, 3)},
+        "tuning": tuning,
         "pair_fracs": {
             "hbm_per_view_bytes": round((fwd_b + bwd_b) * pairs_per_s / HBM_PEAK, 4),
             "hbm_compulsory_bytes": round(comp_b * pairs_per_s / HBM_PEAK, 6),

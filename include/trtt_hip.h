@@ -182,6 +182,29 @@ int trtt_hip_plan_error_flag(trtt_hip_plan_t *plan);
  * axial closed-form kernels with a footprint width bound of W pixels. */
 int trtt_hip_plan_path(const trtt_hip_plan_t *plan);
 
+/* Launch shape of the closed-form axial kernels of a plan (no effect on the
+ * general kernels).  Results do not depend on it beyond FP32 summation order;
+ * the autotuner (yoricktrtt_amd/autotune.py) times the candidates and keeps
+ * the fastest per geometry.  set_tuning reallocates the forward scratch when
+ * the tile or window changes: call it while no launch of the plan is in
+ * flight. */
+typedef struct trtt_hip_tuning {
+  int fwd_tile;            /* 0: 8x4, 1: 4x4, 2: 8x8 voxel columns x 64 k */
+  int bwd_tile;            /* 0: 8x8, 1: 8x4, 2: 4x4 voxel columns x 64 k */
+  int fwd_window_bytes;    /* LDS detector window of a forward wave, 4096..65536 */
+  int fwd_views_per_block; /* views per forward workgroup, 0 = from the grid size */
+} trtt_hip_tuning_t;
+int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *plan, trtt_hip_tuning_t *tuning);
+int trtt_hip_plan_set_tuning(trtt_hip_plan_t *plan, const trtt_hip_tuning_t *tuning);
+
+/* Kernel timing of the axial path: when on, the plan records HIP events on
+ * the launch stream around its main forward kernel (one pair per chunk of
+ * views) and its backward kernel (leave it off under graph capture).
+ * kernel_time waits for the events of the last forward (job 0) or backward
+ * (job 1) call and returns the summed duration in ms and the launch count. */
+int trtt_hip_plan_set_timing(trtt_hip_plan_t *plan, int on);
+int trtt_hip_plan_kernel_time(trtt_hip_plan_t *plan, int job, double *ms, int *launches);
+
 long trtt_hip_plan_nviews(const trtt_hip_plan_t *plan);
 long trtt_hip_plan_volume_size(const trtt_hip_plan_t *plan);   /* nt*nz*ny*nx */
 long trtt_hip_plan_sino_size(const trtt_hip_plan_t *plan);     /* nviews*nv*nu */
@@ -202,7 +225,8 @@ int trtt_hip_tv(const void *x, void *g, void *work, double *f_dev, long nx, long
                 long nt, double weight, double wt, double eps, int precision, void *stream);
 
 /* Samples at the integer nodes of cubic B-spline coefficients, per volume:
- * dst (+)= S src with S = [1/6, 2/3, 1/6] along x, y and z, zero padding
+ * dst (+)= S src with S = [1/6, 2/3, 1/6] along x, y and z (x and y only
+ * when nz == 1: a 2-D image), zero padding
  * (S is symmetric: it is also its own transpose).  src != dst. */
 int trtt_hip_bspline_filter(const void *src, void *dst, long nx, long ny, long nz, long nt,
                             int accumulate, int precision, void *stream);

@@ -174,6 +174,26 @@ def test_tv_and_filter(nt):
         assert abs(fd - gnp[i]) <= 1e-6 * max(1.0, abs(gnp[i]))
 
 
+def test_filter_single_slice_is_2d():
+    """nz == 1: the sampling filter acts along x and y only."""
+    from scipy import ndimage
+
+    from yoricktrtt_amd import _lib
+    from yoricktrtt_amd.reconstruction import _stream, _vp
+
+    _say("2-D filter")
+    lib = _lib.load()
+    x = np.random.default_rng(9).normal(size=(13, 17))
+    xd = torch.tensor(x, device="cuda")
+    out = torch.empty_like(xd)
+    assert lib.trtt_hip_bspline_filter(_vp(xd), _vp(out), 17, 13, 1, 1, 0, _lib.F64,
+                                       _stream(torch)) == 0
+    k1 = np.array([1 / 6, 2 / 3, 1 / 6])
+    ref = ndimage.convolve1d(ndimage.convolve1d(x, k1, axis=0, mode="constant"), k1, axis=1,
+                             mode="constant")
+    assert rel_err(out.cpu().numpy(), ref) < 1e-13
+
+
 def test_vmlmb_gpu_matches_cpu_fp64(oracle):
     """Five VMLM-B iterations on the GPU (FP64 plan) and on the CPU (FP64
     oracle data term) from the same start: same iterates."""
@@ -205,4 +225,36 @@ def test_vmlmb_gpu_matches_cpu_fp64(oracle):
     assert rel_err(rg.x.cpu().numpy(), rc.x.numpy()) < 1e-10
     assert abs(rg.f - rc.f) <= 1e-10 * abs(rc.f)
     assert rg.history[-1] < 0.2 * rg.history[0]
+    P.close()
+
+
+def test_autotune_candidates_and_cache(tmp_path):
+    """Every launch shape reproduces the default projections (checked inside
+    the tuner), the chosen shape is applied and cached, and a second call is
+    a cache hit; out-of-range shapes are refused."""
+    from yoricktrtt_amd import autotune as AT
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("autotune")
+    th = np.arange(16) * (2 * math.pi / 16)
+    geo = cone_geometry(40, 36, 70, 1.0, 80, 72, 1.0, 1.0, th, 90.0, 140.0)
+    P = Projector(geo, model="cubic", precision="f32")
+    assert P.path > 0
+    cache = str(tmp_path / "tuning.json")
+    r = AT.autotune(P, cache=cache, reps=1, windows=(8192, 16384), vpbs=(0, 2))
+    assert not r["cached"] and AT.get_tuning(P) == r["tuning"]
+    assert len(r["rows"]) == 3 * 2 * 2 + 3
+    r2 = AT.autotune(P, cache=cache)
+    assert r2["cached"] and r2["tuning"] == r["tuning"]
+    with pytest.raises(RuntimeError):
+        AT.set_tuning(P, fwd_tile=7)
+    with pytest.raises(RuntimeError):
+        AT.set_tuning(P, fwd_window_bytes=1000)
+    # a tuned plan still matches the oracle-checked default on fresh inputs
+    x = torch.rand(P.vol_shape, device="cuda")
+    a = P.forward(x)
+    AT.set_tuning(P, **r["default"])
+    b = P.forward(x)
+    assert rel_err(a.cpu().numpy(), b.cpu().numpy()) < 1e-5
     P.close()

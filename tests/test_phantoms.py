@@ -1,0 +1,105 @@
+"""Phantoms and analytic data (CPU): the band-integrated ellipse Radon
+transforms against closed forms and against the oracle's algebraic
+projection of the sampled phantom, with the reference's angle convention
+(analytic angle = view angle - pi/2, Ytrtt2D_script_reconstructions.i:46-48)."""
+import math
+
+import numpy as np
+import pytest
+
+from yoricktrtt_amd import phantoms as P
+
+
+def test_parallel_band_integral_closed_forms():
+    # a centred disk of radius R: the bins tile the line, so they sum to the area
+    R, mu = 7.3, 0.8
+    ell = np.array([[0.0, 0.0, R, R, 0.3, mu]])
+    v = np.arange(-20, 21) * 0.5
+    s = P.radon_ellipses_parallel(ell, np.array([0.0, 1.1]), v, 0.5)
+    assert np.allclose(s.sum(axis=1), mu * math.pi * R * R, rtol=1e-7)
+    # one narrow bin at the centre: chord length 2R times the width
+    s = P.radon_ellipses_parallel(ell, [0.4], [0.0], 1e-6)
+    assert abs(s[0, 0] / 1e-6 - 2 * R * mu) < 1e-6
+    # an off-centre rotated ellipse: total mass pi a b mu at every angle
+    ell = np.array([[3.0, -2.0, 5.0, 2.0, 0.7, 1.3]])
+    s = P.radon_ellipses_parallel(ell, np.linspace(0, math.pi, 7), np.arange(-40, 41) * 0.25, 0.25)
+    assert np.allclose(s.sum(axis=1), 1.3 * math.pi * 10.0, rtol=1e-7)
+
+
+def test_fan_equals_parallel_at_large_distance():
+    ell = np.array([[3.0, -2.0, 5.0, 2.0, 0.7, 1.0], [-4.0, 1.0, 2.0, 3.0, 0.1, 0.5]])
+    th = np.array([0.2, 1.3])
+    v = np.arange(-30, 31) * 0.5
+    par = P.radon_ellipses_parallel(ell, th, v, 0.5)
+    fan = P.radon_ellipses_fan(ell, th, v, 0.5, 1e7, 1e7, nodes=16)
+    assert np.abs(fan - par).max() < 2e-3 * np.abs(par).max()
+
+
+@pytest.mark.parametrize("fan", [False, True])
+def test_analytic_vs_oracle_projection(oracle, fan):
+    from yoricktrtt_amd.geometry import fan_geometry_2d, parallel_geometry_2d
+
+    n, nv = 64, 120
+    img, ells = P.ellipses_2d(n, 5, 1)
+    theta = np.arange(18) * (2 * math.pi / 18)
+    vs = 1.5 if fan else 1.0
+    v = (np.arange(nv) - (nv - 1) / 2) * vs
+    if fan:
+        g = fan_geometry_2d(n, n, 1.0, nv, vs, theta, 200.0, 300.0)
+        an = P.radon_ellipses_fan(ells, theta - 0.5 * math.pi, v, vs, 200.0, 300.0)
+    else:
+        g = parallel_geometry_2d(n, n, 1.0, nv, vs, theta)
+        an = P.radon_ellipses_parallel(ells, theta - 0.5 * math.pi, v, vs)
+    alg = np.stack([oracle.sb2d(np.zeros(nv), img, n, n, 1.0, nv, vs, g.xos[k], g.xsd[k], fan, 0)
+                    for k in range(theta.size)])
+    # the sampled phantom differs from the exact ellipses at their edges only
+    err = np.linalg.norm(alg - an) / np.linalg.norm(an)
+    assert err < 0.06
+    # the mirrored convention is far off (the test can tell them apart)
+    assert np.linalg.norm(alg - an[:, ::-1]) / np.linalg.norm(an) > 0.3
+
+
+def test_phantom_generators_seeded():
+    a = P.body_phantom(32, 10000)
+    b = P.body_phantom(32, 10000)
+    c = P.body_phantom(32, 10001)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert a.min() >= 0.0 and a.max() > 0.2
+    img, rows = P.ellipses_2d(128, 10, 1)
+    assert rows.shape == (10, 6)
+    r = np.hypot(rows[:, 0], rows[:, 1])
+    assert np.all(r <= 0.8 * 64 + 1e-9)
+    assert np.all((rows[:, 2:4] >= 0.05 * 64) & (rows[:, 2:4] <= 0.4 * 64))
+    vol = P.ellipsoids_3d((16, 20, 24), 5, 3)
+    assert vol.shape == (16, 20, 24) and vol.max() > 0
+
+
+def test_transmission_noise_model():
+    torch = pytest.importorskip("torch")
+    p = torch.full((200000,), 30.0)
+    g = torch.Generator().manual_seed(5)
+    y, w = P.transmission_noise(p, 1e5, mu_scale=0.02, generator=g)
+    z = 1e5 * math.exp(-0.6)
+    # unbiased to first order and with the model variance 1/(z mu^2)
+    assert abs(float(y.mean()) - 30.0) < 0.01
+    assert abs(float(y.var()) * z * 0.02 ** 2 - 1.0) < 0.03
+    assert abs(float(w.mean()) / (z * 0.02 ** 2) - 1.0) < 0.01
+
+
+def test_lf2d_angle_convention(oracle):
+    """The Long & Fessler source angle is the view angle minus pi/2: the
+    oracle's lf2d_direct of the sampled phantom then matches the analytic
+    fan-beam data (the trapezoid footprint differs at the edges only)."""
+    from oracle.oracle import LF2DParams
+
+    n, s, vs, nv, Rsc, Rsd = 64, 0.5, 0.75, 80, 1000.0, 1500.0
+    img, ells = P.ellipses_2d(n, 5, 1)
+    e = ells.copy()
+    e[:, :4] *= s
+    theta = np.arange(12) * (2 * math.pi / 12)
+    v = (np.arange(nv) - (nv - 1) / 2) * vs
+    an = P.radon_ellipses_fan(e, theta - 0.5 * math.pi, v, vs, Rsc, Rsd)
+    prm = LF2DParams(vs, vs, s, Rsc, Rsd - Rsc, 0.0, 0.0, n, n, nv)
+    alg = np.stack([oracle.lf2d(prm, th - 0.5 * math.pi, 0.0, img, np.zeros(nv), 1, 0)
+                    for th in theta])
+    assert np.linalg.norm(alg - an) / np.linalg.norm(an) < 0.06

@@ -24,6 +24,13 @@ F32, F64 = 0, 1
 PB3D, CB3D, PB2D, FB2D, LF2D = 0, 1, 2, 3, 4
 
 
+class Tuning(C.Structure):
+    """trtt_hip_tuning_t: launch shape of the axial kernels of a plan."""
+
+    _fields_ = [("fwd_tile", C.c_int), ("bwd_tile", C.c_int), ("fwd_window_bytes", C.c_int),
+                ("fwd_views_per_block", C.c_int)]
+
+
 class LF2DParameters(C.Structure):
     """lf2d_parameters_t (LongFessler2D.h:16-28)."""
 
@@ -56,6 +63,10 @@ _SIGS = {
     "trtt_hip_plan_error_flag": ([vp], I),
     "trtt_hip_plan_path": ([vp], I),
     "trtt_hip_plan_nviews": ([vp], L),
+    "trtt_hip_plan_get_tuning": ([vp, C.POINTER(Tuning)], I),
+    "trtt_hip_plan_set_tuning": ([vp, C.POINTER(Tuning)], I),
+    "trtt_hip_plan_set_timing": ([vp, I], I),
+    "trtt_hip_plan_kernel_time": ([vp, I, dp, ip], I),
     "trtt_hip_plan_volume_size": ([vp], L),
     "trtt_hip_plan_sino_size": ([vp], L),
     "trtt_hip_plan_destroy": ([vp], I),

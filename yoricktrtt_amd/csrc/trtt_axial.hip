@@ -677,36 +677,54 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
 
 }  // namespace axial
 
-// Forward tile: 16x16 columns x 64 k; backward tile: 8x8 columns x 64 k.
-// Forward: one wave per workgroup, tile 8x4 columns x 64 k, private 16 KiB
-// window.  Backward: four waves, tile 8x8 columns x 64 k, shared 32 KiB window.
-constexpr int kFwdCX = 8, kFwdCY = 4, kBwdCX = 8, kBwdCY = 8;
+// Tile shapes (columns along x, y; each tile spans 64 k) selectable per
+// plan: forward one wave per workgroup, backward four waves.
+constexpr int kFwdTiles[kNumFwdTiles][2] = {{8, 4}, {4, 4}, {8, 8}};
+constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}};
 constexpr int kAxialRegionBytes = 32768;
-constexpr int kFwdWindowBytes = 16384;
 constexpr int kAxialPmax = 256;  // longest column profile held in LDS
 
-
-int axial_tiles(const DevPlan &P, int job) {
-  const int cx = job == 0 ? kFwdCX : kBwdCX, cy = job == 0 ? kFwdCY : kBwdCY;
+int axial_tiles(const DevPlan &P, int cx, int cy) {
   return ((P.nx + cx - 1) / cx) * ((P.ny + cy - 1) / cy) * ((P.nz + axial::kKT - 1) / axial::kKT);
+}
+
+int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu) {
+  const int *t = job == 0 ? kFwdTiles[tu.fwd_tile] : kBwdTiles[tu.bwd_tile];
+  return axial_tiles(P, t[0], t[1]);
 }
 
 // Scratch of the forward reduction: one detector window per (view, tile)
 // for `chunk` views, plus one header per slot.
-size_t axial_scratch_bytes(const DevPlan &P, int chunk) {
-  const size_t slots = size_t(chunk) * size_t(axial_tiles(P, 0));
-  return slots * size_t(kFwdWindowBytes) + slots * sizeof(int4);
+size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
+  const size_t slots = size_t(chunk) * size_t(axial_tiles(P, 0, tu));
+  return slots * size_t(tu.fwd_window_bytes) + slots * sizeof(int4);
+}
+
+template <bool CONE, int W, int CX, int CY, typename T>
+void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *volk, T *out,
+                int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain) {
+  hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, T>), grid, dim3(64), lds_bytes, st, P, volk, out,
+                     vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
+}
+
+template <bool CONE, int W, int CX, int CY, typename T>
+void launch_bwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *in, T *out,
+                const int *seg_begin, int atomic_out, int lds_cap, int pmax) {
+  hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, T>), grid, dim3(axial::kThreads), lds_bytes, st,
+                     P, in, out, seg_begin, atomic_out, lds_cap, pmax);
 }
 
 template <bool CONE, int W, typename T>
 hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
                           const int *seg_begin, int nrows, int views_per_block, int atomic_out,
                           int pmax, void *scratch, void *hdr, int chunk, void *volk_buf,
-                          int plain, hipStream_t st) {
+                          int plain, const AxialTuning &tu, hipEvent_t *ev, hipStream_t st) {
+  // ev (optional): pairs of events around each main kernel launch
+  int npair = 0;
   if (job == 0) {
-    const int wcap = kFwdWindowBytes / int(sizeof(T));
+    const int wcap = tu.fwd_window_bytes / int(sizeof(T));
     const int lds_bytes = (wcap + pmax) * int(sizeof(T));
-    const unsigned ntiles = unsigned(axial_tiles(P, 0));
+    const unsigned ntiles = unsigned(axial_tiles(P, 0, tu));
     const int npt = ((P.nu + 63) / 64) * ((P.nv + 15) / 16);
     T *volk = static_cast<T *>(volk_buf);
     hipLaunchKernelGGL(axial::to_k_fastest<T>, dim3(((P.nx + 31) / 32) * P.ny, (P.nz + 63) / 64, P.nt),
@@ -715,9 +733,25 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     for (int vb = 0; vb < P.nviews; vb += chunk) {
       const int ve = vb + chunk < P.nviews ? vb + chunk : P.nviews;
       dim3 grid(ntiles, (ve - vb + views_per_block - 1) / views_per_block);
-      hipLaunchKernelGGL((axial::fwd<CONE, W, kFwdCX, kFwdCY, T>), grid, dim3(64), lds_bytes, st, P, static_cast<const T *>(volk), static_cast<T *>(out),
-                         views_per_block, wcap, pmax, vb, ve, static_cast<T *>(scratch),
-                         static_cast<int4 *>(hdr), plain);
+      auto *o = static_cast<T *>(out);
+      auto *sc = static_cast<T *>(scratch);
+      auto *hd = static_cast<int4 *>(hdr);
+      if (ev) hipEventRecord(ev[2 * npair], st);
+      switch (tu.fwd_tile) {
+        case 1:
+          launch_fwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
+                                       vb, ve, sc, hd, plain);
+          break;
+        case 2:
+          launch_fwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
+                                       vb, ve, sc, hd, plain);
+          break;
+        default:
+          launch_fwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
+                                       vb, ve, sc, hd, plain);
+      }
+      if (ev) hipEventRecord(ev[2 * npair + 1], st);
+      ++npair;
       hipLaunchKernelGGL(axial::fwd_reduce<T>, dim3(npt, ve - vb), dim3(axial::kThreads), 0, st,
                          P, static_cast<const T *>(scratch), static_cast<const int4 *>(hdr),
                          static_cast<T *>(out), vb, int(ntiles), wcap);
@@ -725,10 +759,24 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
   } else {
     const int lds_cap = kAxialRegionBytes / int(sizeof(T));
     const int lds_bytes = kAxialRegionBytes + axial::kWaves * pmax * int(sizeof(T));
-    dim3 grid(unsigned(axial_tiles(P, 1)), nrows);
-    hipLaunchKernelGGL((axial::bwd<CONE, W, kBwdCX, kBwdCY, T>), grid, dim3(axial::kThreads),
-                       lds_bytes, st, P, static_cast<const T *>(in), static_cast<T *>(out),
-                       seg_begin, atomic_out, lds_cap, pmax);
+    dim3 grid(unsigned(axial_tiles(P, 1, tu)), nrows);
+    const auto *i = static_cast<const T *>(in);
+    auto *o = static_cast<T *>(out);
+    if (ev) hipEventRecord(ev[0], st);
+    switch (tu.bwd_tile) {
+      case 1:
+        launch_bwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
+                                     pmax);
+        break;
+      case 2:
+        launch_bwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
+                                     pmax);
+        break;
+      default:
+        launch_bwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
+                                     pmax);
+    }
+    if (ev) hipEventRecord(ev[1], st);
   }
   return hipGetLastError();
 }
@@ -736,11 +784,16 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
 hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
-                        void *volk, int plain, hipStream_t st) {
+                        void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
+                        hipStream_t st) {
+  if (tu.fwd_tile < 0 || tu.fwd_tile >= kNumFwdTiles || tu.bwd_tile < 0 ||
+      tu.bwd_tile >= kNumBwdTiles)
+    return hipErrorInvalidValue;
 #define AX_CASE(C, WW, TT)                                                                   \
   if (cone == C && w == WW && f64 == (sizeof(TT) == 8))                                     \
     return launch_axial_t<C, WW, TT>(P, job, in, out, seg_begin, nrows, views_per_block,   \
-                                     atomic_out, pmax, scratch, hdr, chunk, volk, plain, st);
+                                     atomic_out, pmax, scratch, hdr, chunk, volk, plain, tu, ev, \
+                                     st);
   AX_CASE(true, 9, float)
   AX_CASE(true, 13, float)
   AX_CASE(false, 9, float)

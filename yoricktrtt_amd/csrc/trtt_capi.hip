@@ -21,9 +21,10 @@ hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long
 hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
-                        void *volk, int plain, hipStream_t st);
-int axial_tiles(const DevPlan &P, int job);
-size_t axial_scratch_bytes(const DevPlan &P, int chunk);
+                        void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
+                        hipStream_t st);
+int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu);
+size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
 }  // namespace trtt
 
 using trtt::ViewConst;
@@ -339,6 +340,8 @@ void release(trtt_hip_plan *p) {
   hipFree(p->d_scratch);
   hipFree(p->d_hdr);
   hipFree(p->d_volk);
+  for (hipEvent_t e : p->ev_fwd) hipEventDestroy(e);
+  for (hipEvent_t e : p->ev_bwd) hipEventDestroy(e);
   delete p;
 }
 
@@ -519,18 +522,23 @@ trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, lon
 // Scratch of the axial forward: windows of `chunk` views at a time, at most
 // 4 GiB (views are processed in chunks that fit).
 bool alloc_axial_scratch(trtt_hip_plan *p) {
-  const size_t per_view = trtt::axial_scratch_bytes(p->dev, 1);
+  hipFree(p->d_scratch);
+  hipFree(p->d_hdr);
+  p->d_scratch = p->d_hdr = nullptr;
+  const size_t per_view = trtt::axial_scratch_bytes(p->dev, 1, p->tune);
   const size_t budget = size_t(4) << 30;
   long chunk = long(budget / per_view);
   chunk = std::max(1L, std::min(chunk, p->nviews));
   p->axial_chunk = int(chunk);
-  const size_t slots = size_t(chunk) * size_t(trtt::axial_tiles(p->dev, 0));
+  const size_t slots = size_t(chunk) * size_t(trtt::axial_tiles(p->dev, 0, p->tune));
   const size_t win_bytes = per_view * size_t(chunk) - slots * sizeof(int4);
   if (!hip_ok(hipMalloc(&p->d_scratch, win_bytes), "hipMalloc scratch")) return false;
   if (!hip_ok(hipMalloc(&p->d_hdr, slots * sizeof(int4)), "hipMalloc scratch")) return false;
   // k-fastest copy of the volume series read by the forward
-  const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
-  if (!hip_ok(hipMalloc(&p->d_volk, vbytes), "hipMalloc scratch")) return false;
+  if (!p->d_volk) {
+    const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
+    if (!hip_ok(hipMalloc(&p->d_volk, vbytes), "hipMalloc scratch")) return false;
+  }
   return true;
 }
 
@@ -539,13 +547,27 @@ hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hip
   const int nrows = int(p->h_seg_begin.size()) - 1;
   const bool f64 = p->precision == TRTT_HIP_F64;
   if (p->axial_w > 0) {
-    const long ntiles = trtt::axial_tiles(p->dev, job);
+    const long ntiles = trtt::axial_tiles(p->dev, job, p->tune);
     long vpb = (ntiles * p->nviews) / kTargetBlocks;
     vpb = std::max(1L, std::min(vpb, 8L));
+    if (job == 0 && p->tune.fwd_views_per_block > 0) vpb = p->tune.fwd_views_per_block;
+    hipEvent_t *ev = nullptr;
+    if (p->timing) {
+      std::vector<hipEvent_t> &v = job == 0 ? p->ev_fwd : p->ev_bwd;
+      const size_t need = job == 0 ? 2 * size_t((p->nviews + p->axial_chunk - 1) / p->axial_chunk)
+                                   : 2;
+      while (v.size() < need) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return hipErrorOutOfMemory;
+        v.push_back(e);
+      }
+      ev = v.data();
+      (job == 0 ? p->fwd_launches : p->bwd_launches) = int(need / 2);
+    }
     return trtt::launch_axial(p->kind == trtt::CB3D, p->axial_w, f64, p->dev, job, in, out,
                               p->d_seg_begin, nrows, int(vpb), nrows > 1 ? 1 : 0, p->axial_pmax,
                               p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
-                              p->axial_plain ? 1 : 0, st);
+                              p->axial_plain ? 1 : 0, p->tune, ev, st);
   }
   if (job == 0)
     return trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, 0, in, out,
@@ -660,6 +682,68 @@ int trtt_hip_plan_error_flag(trtt_hip_plan_t *p) {
 }
 
 int trtt_hip_plan_path(const trtt_hip_plan_t *p) { return p ? p->axial_w : -1; }
+
+int trtt_hip_plan_set_timing(trtt_hip_plan_t *p, int on) {
+  if (!p) return fail("null argument");
+  p->timing = on != 0;
+  p->fwd_launches = p->bwd_launches = 0;
+  return TRTT_SUCCESS;
+}
+
+int trtt_hip_plan_kernel_time(trtt_hip_plan_t *p, int job, double *ms, int *launches) {
+  if (!p || !ms) return fail("null argument");
+  const std::vector<hipEvent_t> &v = job == 0 ? p->ev_fwd : p->ev_bwd;
+  const int n = job == 0 ? p->fwd_launches : p->bwd_launches;
+  if (!p->timing || n == 0) return fail("no timed launch (enable timing on an axial plan)");
+  double total = 0.0;
+  for (int k = 0; k < n; ++k) {
+    float t = 0.0f;
+    if (!hip_ok(hipEventSynchronize(v[2 * k + 1]), "hipEventSynchronize") ||
+        !hip_ok(hipEventElapsedTime(&t, v[2 * k], v[2 * k + 1]), "hipEventElapsedTime"))
+      return TRTT_FAILURE;
+    total += t;
+  }
+  *ms = total;
+  if (launches) *launches = n;
+  return TRTT_SUCCESS;
+}
+
+int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *p, trtt_hip_tuning_t *t) {
+  if (!p || !t) return fail("null argument");
+  t->fwd_tile = p->tune.fwd_tile;
+  t->bwd_tile = p->tune.bwd_tile;
+  t->fwd_window_bytes = p->tune.fwd_window_bytes;
+  t->fwd_views_per_block = p->tune.fwd_views_per_block;
+  return TRTT_SUCCESS;
+}
+
+int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
+  if (!p || !t) return fail("null argument");
+  if (t->fwd_tile < 0 || t->fwd_tile >= trtt::kNumFwdTiles || t->bwd_tile < 0 ||
+      t->bwd_tile >= trtt::kNumBwdTiles)
+    return fail("tile shape index out of range");
+  if (t->fwd_window_bytes < 4096 || t->fwd_window_bytes > 65536 || t->fwd_window_bytes % 256)
+    return fail("forward window must be 4096..65536 bytes, a multiple of 256");
+  if (t->fwd_views_per_block < 0 || t->fwd_views_per_block > 64)
+    return fail("views per block must be 0..64");
+  if (p->axial_w <= 0) return TRTT_SUCCESS;  // general kernels: nothing to tune
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return TRTT_FAILURE;
+  const trtt::AxialTuning old = p->tune;
+  p->tune.fwd_tile = t->fwd_tile;
+  p->tune.bwd_tile = t->bwd_tile;
+  p->tune.fwd_window_bytes = t->fwd_window_bytes;
+  p->tune.fwd_views_per_block = t->fwd_views_per_block;
+  if (old.fwd_tile != p->tune.fwd_tile || old.fwd_window_bytes != p->tune.fwd_window_bytes) {
+    // hipFree waits for the device: no launch of this plan is in flight
+    if (!alloc_axial_scratch(p)) {
+      p->tune = old;
+      alloc_axial_scratch(p);
+      return fail("scratch allocation for the new tuning failed");
+    }
+  }
+  return TRTT_SUCCESS;
+}
 
 long trtt_hip_plan_nviews(const trtt_hip_plan_t *p) { return p ? p->nviews : 0; }
 long trtt_hip_plan_volume_size(const trtt_hip_plan_t *p) {

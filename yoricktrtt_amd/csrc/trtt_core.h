@@ -61,6 +61,16 @@ struct DevPlan {
   double lf_vs, lf_ps, lf_R, lf_scale, lf_size;
 };
 
+// Launch shape of the axial kernels of a plan (trtt_hip_plan_set_tuning).
+constexpr int kNumFwdTiles = 3;  // 8x4, 4x4, 8x8 columns x 64 k
+constexpr int kNumBwdTiles = 3;  // 8x8, 8x4, 4x4 columns x 64 k
+struct AxialTuning {
+  int fwd_tile = 0;
+  int bwd_tile = 0;
+  int fwd_window_bytes = 16384;   // private LDS window of a forward wave
+  int fwd_views_per_block = 0;    // 0: from the grid size
+};
+
 }  // namespace trtt
 
 // Host-side plan (opaque to C callers).
@@ -95,4 +105,10 @@ struct trtt_hip_plan {
   void *d_scratch = nullptr;  // forward windows, one per (view, tile)
   void *d_hdr = nullptr;      // int4 (ul, vl, width, height) per window
   void *d_volk = nullptr;     // volume series with k fastest (forward input)
+  trtt::AxialTuning tune;
+  // optional HIP events around the main kernels of the last forward (one
+  // pair per view chunk) and backward (one pair) of the axial path
+  bool timing = false;
+  std::vector<hipEvent_t> ev_fwd, ev_bwd;
+  int fwd_launches = 0, bwd_launches = 0;
 };

@@ -9,8 +9,8 @@
 // (the reference extends t periodically, Ytrtt4D_optim.i:225-255).
 //
 // B-spline samples of cubic coefficients at the integer nodes: separable
-// [1/6, 2/3, 1/6] along x, y, z with zero padding (symmetric, so the same
-// filter is its own transpose).
+// [1/6, 2/3, 1/6] along x, y, z (x, y only for a single slice) with zero
+// padding (symmetric, so the same filter is its own transpose).
 #include "trtt_core.h"
 
 namespace trtt {
@@ -107,7 +107,9 @@ __global__ __launch_bounds__(kThreads) void bspline_filter(const T *__restrict__
     const long k = r / (nx * ny), r2 = r - k * nx * ny, j = r2 / nx, i = r2 - j * nx;
     const T *s = src + t * nvox;
     T acc = T(0);
-    for (int c = -1; c <= 1; ++c) {
+    // a single slice (nz == 1) is a 2-D image: no filtering along z
+    const int cz = nz > 1 ? 1 : 0;
+    for (int c = -cz; c <= cz; ++c) {
       const long kk = k + c;
       if (kk < 0 || kk >= nz) continue;
       for (int b = -1; b <= 1; ++b) {
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(kThreads) void bspline_filter(const T *__restrict__
           if (ii < 0 || ii >= nx) continue;
           row += wv[a + 1] * s[(kk * ny + jj) * nx + ii];
         }
-        acc += wv[c + 1] * wv[b + 1] * row;
+        acc += (cz ? wv[c + 1] : T(1)) * wv[b + 1] * row;
       }
     }
     if (accumulate)

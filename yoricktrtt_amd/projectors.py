@@ -292,6 +292,21 @@ class Projector:
                                                self._stream(stream)) != 0:
             raise RuntimeError(_lib.last_error())
 
+    def set_timing(self, on: bool = True) -> None:
+        """Record HIP events around the main axial kernels of each call
+        (not under graph capture)."""
+        if self.lib.trtt_hip_plan_set_timing(self.handle, int(on)) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def kernel_time(self, job: int):
+        """(ms, launches) of the main kernel of the last forward (job 0) or
+        backward (job 1) call, from the plan's HIP events."""
+        ms = C.c_double(0.0)
+        n = C.c_int(0)
+        if self.lib.trtt_hip_plan_kernel_time(self.handle, int(job), C.byref(ms), C.byref(n)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return ms.value, n.value
+
     @property
     def path(self) -> int:
         """0: general kernels; W > 0: axial closed-form kernels (width bound W)."""

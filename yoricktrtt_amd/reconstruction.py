@@ -90,7 +90,8 @@ class DataTerm:
 
 class TotalVariation:
     """w * sum (sqrt(eps^2 + |D S x|^2) - eps), S the B-spline sampling
-    (or the identity when ``samples`` is False)."""
+    (or the identity when ``samples`` is False), for an image [ny][nx], a
+    volume [nz][ny][nx] or a series [nt][nz][ny][nx]."""
 
     def __init__(self, shape, weight, eps, wt=1.0, samples=True, dtype=None, device=None):
         import torch
@@ -98,7 +99,7 @@ class TotalVariation:
         self.torch = torch
         self.shape = tuple(shape)
         self.nt = shape[0] if len(shape) == 4 else 1
-        self.nz, self.ny, self.nx = shape[-3:]
+        self.nz, self.ny, self.nx = ((1,) + self.shape)[-3:] if len(shape) == 2 else shape[-3:]
         self.weight, self.eps, self.wt = float(weight), float(eps), float(wt)
         self.samples = samples
         dtype = dtype or torch.float32

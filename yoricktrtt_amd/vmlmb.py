@@ -34,6 +34,7 @@ class Result:
     neval: int
     status: str
     history: list = field(default_factory=list)
+    pairs: list = field(default_factory=list)  # L-BFGS memory (s, y, 1/s'y) at exit
 
 
 def _project(x, lower, upper):
@@ -45,14 +46,16 @@ def _project(x, lower, upper):
 
 
 def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol=0.0,
-          grtol=0.0, frtol=0.0, ftol=1e-4, callback=None):
+          grtol=0.0, frtol=0.0, ftol=1e-4, fstop=None, callback=None, pairs=None):
     """Minimise f over lower <= x <= upper.
 
     ``fg(x, g)`` returns f(x) as a float and writes the gradient into g.
     Stops after ``maxiter`` iterations, when the projected gradient norm falls
-    below max(gatol, grtol |pg_0|), or when the relative decrease of f over
-    an iteration is below ``frtol``.  ``callback(it, x, f)`` is called after
-    each iteration.
+    below max(gatol, grtol |pg_0|), when the relative decrease of f over
+    an iteration is below ``frtol``, or once f <= ``fstop``.
+    ``callback(it, x, f)`` is called after each iteration.  ``pairs`` seeds
+    the L-BFGS memory (e.g. the ``pairs`` of an earlier run on the same
+    objective, so a restart from a new point keeps its curvature model).
     """
     import torch
 
@@ -60,13 +63,16 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
     g = torch.empty_like(x)
     f = fg(x, g)
     neval = 1
-    pairs = []  # (s, y, rho)
+    pairs = list(pairs[-mem:]) if pairs else []  # (s, y, rho)
     history = [f]
     g0norm = None
     status = "maxiter"
     xn = torch.empty_like(x)
     gn = torch.empty_like(x)
     it = 0
+    if fstop is not None and f <= fstop:
+        status = "fstop"
+        maxiter = 0
     while it < maxiter:
         free = torch.ones_like(x, dtype=torch.bool)
         if lower is not None:
@@ -139,7 +145,10 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
         if frtol > 0.0 and abs(fprev - f) <= frtol * max(abs(f), abs(fprev)):
             status = "frtol"
             break
+        if fstop is not None and f <= fstop:
+            status = "fstop"
+            break
         if maxeval is not None and neval >= maxeval:
             status = "maxeval"
             break
-    return Result(x=x, f=f, niter=it, neval=neval, status=status, history=history)
+    return Result(x=x, f=f, niter=it, neval=neval, status=status, history=history, pairs=pairs)
