@@ -60,6 +60,30 @@ def phantom(seed=3, n_ell=30):
     return vol
 
 
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "axial_fwd_cone256_traffic.txt")
+
+
+def measured_traffic(tuning, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a 1 GiB copy),
+    when it was measured with the launch shape in use; else None."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    vals, shape = {}, None
+    for line in text.splitlines():
+        if line.startswith("# tuning:"):
+            shape = dict(kv.split("=") for kv in line[len("# tuning:"):].replace(",", " ").split())
+        elif "=" in line and not line.startswith("#"):
+            k, v = line.split("=")
+            vals[k.strip()] = float(v)
+    if not tuning or not shape or any(int(shape[k]) != int(tuning[k]) for k in shape):
+        return None
+    key = "bwd_traffic_bytes_per_launch" if kernel == "axial::bwd" else "fwd_traffic_bytes_per_launch"
+    return int(vals[key]) if key in vals else None
+
+
 def algorithmic_bytes(esz=4):
     """Per-view-operator bytes: each view reads the volume and writes its
     projection (forward), or reads its projection and read-modify-writes the
@@ -115,6 +139,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("BENCH_CPU_THREADS", 16)))
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run an elementwise copy of 1 GiB (HBM counter calibration)")
     ap.add_argument("--tune-cache", default=os.path.join(HERE, "yoricktrtt_amd", "tuning",
                                                          "tuning.json"))
     args = ap.parse_args()
@@ -127,6 +153,15 @@ def main():
     from yoricktrtt_amd.projectors import Projector
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        # reads 1 GiB and writes 1 GiB: the byte counters of this kernel
+        # calibrate FETCH_SIZE / WRITE_SIZE of the projector kernels
+        a = torch.rand(1 << 28, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.add(a, 0.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     geo = geometry()
     P = Projector(geo, model=args.model, precision="f32")
     print(f"bench: plan ready ({NVIEWS} views)", file=sys.stderr, flush=True)
@@ -223,7 +258,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": None,
+            "traffic": None,  # filled below from the rocprofv3 counters
             "bytes_model": "per-view-operator bytes (each view reads the volume; transpose "
                            "read-modify-writes it)",
         },
@@ -236,6 +271,7 @@ def main():
             "hbm_compulsory_bytes": round(comp_b * pairs_per_s / HBM_PEAK, 6),
         },
     }
+    result["roofline"]["traffic"] = measured_traffic(tuning, result["roofline"]["kernel"])
     if not args.no_cpu:
         print("bench: timing the reference C plugin on the host", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_reference_baseline(geo, args.cpu_threads)

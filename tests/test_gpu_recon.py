@@ -244,7 +244,7 @@ def test_autotune_candidates_and_cache(tmp_path):
     cache = str(tmp_path / "tuning.json")
     r = AT.autotune(P, cache=cache, reps=1, windows=(8192, 16384), vpbs=(0, 2))
     assert not r["cached"] and AT.get_tuning(P) == r["tuning"]
-    assert len(r["rows"]) == 3 * 2 * 2 + 3
+    assert len(r["rows"]) == 3 * 2 * 2 + 9
     r2 = AT.autotune(P, cache=cache)
     assert r2["cached"] and r2["tuning"] == r["tuning"]
     with pytest.raises(RuntimeError):

@@ -28,7 +28,7 @@ class Tuning(C.Structure):
     """trtt_hip_tuning_t: launch shape of the axial kernels of a plan."""
 
     _fields_ = [("fwd_tile", C.c_int), ("bwd_tile", C.c_int), ("fwd_window_bytes", C.c_int),
-                ("fwd_views_per_block", C.c_int)]
+                ("fwd_views_per_block", C.c_int), ("bwd_window_bytes", C.c_int)]
 
 
 class LF2DParameters(C.Structure):
@@ -115,11 +115,14 @@ def header_functions(path: str = HEADER) -> list[str]:
     return sorted(names)
 
 
-def load(path: str = LIB_PATH) -> C.CDLL:
-    """Load (once) and type the library; raises when it has not been built."""
+def load(path: str = None) -> C.CDLL:
+    """Load (once) and type the library; raises when it has not been built.
+    ``TRTT_HIP_LIB`` names another build of the same library (used to compare
+    compiler settings)."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("TRTT_HIP_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g;"
                            " g.build()'` (there is no CPU fallback)")

@@ -9,8 +9,9 @@ table keyed by the geometry signature (kind, sizes, view count, footprint
 width bound, precision) and the GPU name, so a geometry is tuned once per
 machine.
 
-Fixed by design: results differ between launch shapes only in FP32
-summation order (checked here to 1e-5 relative before a candidate is timed).
+Results differ between launch shapes only in FP32 summation order
+(checked here to 1e-5 relative before a candidate is timed); a candidate
+whose first run takes over twice the best time so far is not timed further.
 """
 from __future__ import annotations
 
@@ -23,8 +24,11 @@ from . import _lib
 
 FWD_TILES = {0: "8x4", 1: "4x4", 2: "8x8"}
 BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4"}
-WINDOWS = (8192, 12288, 16384, 24576)
+# LDS window sizes tried per forward tile (a window too small for a tile's
+# detector footprint sends its voxels down the slow global-atomic path)
+WINDOWS = {0: (12288, 16384, 24576), 1: (8192, 12288, 16384), 2: (24576, 32768)}
 VPBS = (0, 1, 2, 4)
+BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384)}
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "yoricktrtt_amd", "tuning.json")
 
 
@@ -52,10 +56,12 @@ def set_tuning(P, **kw):
 
 def candidates(windows=WINDOWS, vpbs=VPBS):
     """Forward and backward candidates, tuned independently (the two
-    kernels share no launch parameter)."""
+    kernels share no launch parameter).  ``windows`` maps a forward tile to
+    its window sizes, or is one tuple for every tile."""
+    per_tile = windows if isinstance(windows, dict) else {t: windows for t in FWD_TILES}
     fwd = [dict(fwd_tile=t, fwd_window_bytes=wb, fwd_views_per_block=v)
-           for t, wb, v in itertools.product(FWD_TILES, windows, vpbs)]
-    bwd = [dict(bwd_tile=t) for t in BWD_TILES]
+           for t in FWD_TILES for wb, v in itertools.product(per_tile[t], vpbs)]
+    bwd = [dict(bwd_tile=t, bwd_window_bytes=wb) for t in BWD_TILES for wb in BWD_WINDOWS[t]]
     return fwd, bwd
 
 
@@ -76,7 +82,12 @@ def _store_cache(path, table):
 
 
 def device_name(torch) -> str:
-    return torch.cuda.get_device_name(torch.cuda.current_device())
+    """Architecture and CU count of the current GPU (the marketing name is
+    not reliable across driver installs)."""
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    arch = getattr(p, "gcnArchName", "") or p.name
+    return f"{arch.split(':')[0]}x{p.multi_processor_count}"
+
 
 
 def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vpbs=VPBS,
@@ -108,9 +119,14 @@ def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vp
     scale_f = float(ref_f.abs().max())
     scale_b = float(ref_b.abs().max())
 
-    def timed(fn):
+    def once(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         fn()
         torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def timed(fn):
         ts = []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -124,10 +140,13 @@ def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vp
     rows = []
     for c in fwd_c:
         set_tuning(P, **c)
-        out = P.forward(x, sino.zero_())
-        err = float((out - ref_f).abs().max()) / scale_f
+        t1 = once(lambda: P.forward(x, sino.zero_()))
+        err = float((sino - ref_f).abs().max()) / scale_f
         if err > 1e-5:
             raise RuntimeError(f"tuning {c} changes the forward projection ({err:.2e})")
+        if best_f is not None and t1 > 2.0 * best_f[1]:
+            rows.append(dict(kind="fwd", **c, ms=1e3 * t1, dropped=True))
+            continue
         t = timed(lambda: P.forward(x, sino.zero_()))
         rows.append(dict(kind="fwd", **c, ms=1e3 * t))
         if log:
@@ -137,10 +156,13 @@ def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vp
     set_tuning(P, **best_f[0])
     for c in bwd_c:
         set_tuning(P, **c)
-        out = P.backward(y, vol.zero_())
-        err = float((out - ref_b).abs().max()) / scale_b
+        t1 = once(lambda: P.backward(y, vol.zero_()))
+        err = float((vol - ref_b).abs().max()) / scale_b
         if err > 1e-5:
             raise RuntimeError(f"tuning {c} changes the back projection ({err:.2e})")
+        if best_b is not None and t1 > 2.0 * best_b[1]:
+            rows.append(dict(kind="bwd", **c, ms=1e3 * t1, dropped=True))
+            continue
         t = timed(lambda: P.backward(y, vol.zero_()))
         rows.append(dict(kind="bwd", **c, ms=1e3 * t))
         if log:

@@ -142,10 +142,10 @@ __device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int 
 }
 
 template <typename T>
-__device__ __forceinline__ void voxel(const DevPlan &P, const ViewConst &g, const Col<T> &c, int k,
+__device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T> &c, int k,
                                       Vox<T> &v) {
   v.uc = c.uc0 + c.duc * double(k);
-  const T uca12 = T((v.uc - g.a12) * P.u_scl);
+  const T uca12 = T((v.uc - a12) * P.u_scl);
   const T du = fsqrt(T(1) + uca12 * uca12 * c.inv_a);
   const T hu = c.hu0 * du;
   v.su = frcp(hu);
@@ -254,8 +254,8 @@ template <typename T>
 __device__ __forceinline__ void column_hull(const DevPlan &P, const ViewConst &g, const Col<T> &c,
                                             int kf, int kl, int &lo, int &hi) {
   Vox<T> a, b;
-  voxel(P, g, c, kf, a);
-  voxel(P, g, c, kl, b);
+  voxel(P, g.a12, c, kf, a);
+  voxel(P, g.a12, c, kl, b);
   lo = max(0, min(a.ul, b.ul) - 1);
   hi = min(P.nu, max(a.ur, b.ur) + 1);
 }
@@ -346,8 +346,29 @@ __device__ __forceinline__ void block_window(const ColLane<T> &cl, int *red, int
 // on distinct window columns.  The window goes to a scratch slot; fwd_reduce
 // sums the slots covering each detector pixel.
 // ---------------------------------------------------------------------------
+// Occupancy targets (waves per SIMD) for the register allocator, 0 leaves
+// the allocation to the compiler.  Measured on the 256^3 cone benchmark:
+// forward 5 (96 VGPRs) beats 4 (106) and 6 (80, spills); backward 6 (78)
+// beats 4 (127) and 5 (90).
+#ifndef TRTT_AXIAL_FWD_WAVES
+#define TRTT_AXIAL_FWD_WAVES 5
+#endif
+#ifndef TRTT_AXIAL_BWD_WAVES
+#define TRTT_AXIAL_BWD_WAVES 6
+#endif
+#if TRTT_AXIAL_FWD_WAVES > 0
+#define TRTT_FWD_OCC __attribute__((amdgpu_waves_per_eu(TRTT_AXIAL_FWD_WAVES)))
+#else
+#define TRTT_FWD_OCC
+#endif
+#if TRTT_AXIAL_BWD_WAVES > 0
+#define TRTT_BWD_OCC __attribute__((amdgpu_waves_per_eu(TRTT_AXIAL_BWD_WAVES)))
+#else
+#define TRTT_BWD_OCC
+#endif
+
 template <bool CONE, int W, int CX, int CY, typename T>
-__global__ __launch_bounds__(64) void fwd(DevPlan P, const T *__restrict__ vol,
+__global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__restrict__ vol,
                                           T *__restrict__ sino, int views_per_block, int wcap,
                                           int pmax, int view_base, int view_end,
                                           T *__restrict__ scratch, int4 *__restrict__ hdr,
@@ -371,6 +392,13 @@ __global__ __launch_bounds__(64) void fwd(DevPlan P, const T *__restrict__ vol,
     const ViewConst &g = P.views[view];
     T *out = sino + long(view) * npix;
     const long slot = long(view - view_base) * ntiles + blockIdx.x;
+    // per-view constants of the column loop, read once (the loop's global
+    // stores would otherwise force a reload per column)
+    const double a12 = g.a12;
+    const T w0 = T(g.w0), w1 = T(g.w1);
+    const bool two = g.w1 != 0.0;
+    const T *vol0 = vol + long(g.t0) * P.vox_per_phase;
+    const T *vol1 = vol + long(g.t1) * P.vox_per_phase;
 
     ColLane<T> cl;
     column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
@@ -402,12 +430,12 @@ __global__ __launch_bounds__(64) void fwd(DevPlan P, const T *__restrict__ vol,
         Vox<T> v;
         bool act = false;
         if (kin) {
-          voxel(P, g, c, k, v);
+          voxel(P, a12, c, k, v);
           act = v.ul < v.ur;
           // volume copy with k fastest: the 64 lanes read 64 consecutive words
           const long idx = (long(j) * P.nx + i) * P.nz + k;
-          T x = T(g.w0) * vol[long(g.t0) * P.vox_per_phase + idx];
-          if (g.w1 != 0.0) x += T(g.w1) * vol[long(g.t1) * P.vox_per_phase + idx];
+          T x = w0 * vol0[idx];
+          if (two) x += w1 * vol1[idx];
           val = x * v.rho;
         }
         if (use_prof) {
@@ -494,37 +522,79 @@ __global__ __launch_bounds__(kThreads) void to_k_fastest(const T *__restrict__ v
 }
 
 // Sum of the scratch windows over the tiles, per detector pixel:
-// sino[view][v][u] += sum_t window_t(u, v).  Pixel tile 64 (u) x 16 (v).
+// sino[view][v][u] += sum_t window_t(u, v).  Pixel block 64 (u) x 16 (v).
+// bin_windows first lists, per (view, pixel block), the tiles whose window
+// overlaps the block; a block whose list overflowed scans every header.
+constexpr int kRedU = 64, kRedV = 16;
+constexpr int kBinCap = 2048;  // tiles listed per (view, pixel block)
+
+__global__ __launch_bounds__(kThreads) void zero_ints(int *__restrict__ p, long n) {
+  for (long i = long(blockIdx.x) * kThreads + threadIdx.x; i < n; i += long(gridDim.x) * kThreads)
+    p[i] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void bin_windows(const int4 *__restrict__ hdr, int ntiles,
+                                                        int nviews, int npu, int npv,
+                                                        int *__restrict__ count,
+                                                        int *__restrict__ list) {
+  const long i = long(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= long(ntiles) * nviews) return;
+  const int vloc = int(i / ntiles), t = int(i - long(vloc) * ntiles);
+  const int4 r = hdr[i];
+  if (r.z <= 0 || r.w <= 0) return;
+  const int nb = npu * npv;
+  const int pu0 = max(r.x / kRedU, 0), pu1 = min((r.x + r.z - 1) / kRedU, npu - 1);
+  const int pv0 = max(r.y / kRedV, 0), pv1 = min((r.y + r.w - 1) / kRedV, npv - 1);
+  for (int pv = pv0; pv <= pv1; ++pv)
+    for (int pu = pu0; pu <= pu1; ++pu) {
+      const long b = long(vloc) * nb + pv * npu + pu;
+      const int idx = atomicAdd(count + b, 1);
+      if (idx < kBinCap) list[b * kBinCap + idx] = t;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__restrict__ scratch,
                                                        const int4 *__restrict__ hdr,
+                                                       const int *__restrict__ count,
+                                                       const int *__restrict__ list,
                                                        T *__restrict__ sino, int view_base,
                                                        int ntiles, int cap) {
-  __shared__ int hits[kThreads];
+  __shared__ int hit_t[kThreads];
+  __shared__ int4 hit_r[kThreads];
   __shared__ int nhit;
-  const int npu = (P.nu + 63) / 64;
+  const int npu = (P.nu + kRedU - 1) / kRedU, npv = (P.nv + kRedV - 1) / kRedV;
   const int pu = blockIdx.x % npu, pv = blockIdx.x / npu;
   const int vloc = blockIdx.y;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int u = pu * 64 + tx, v0 = pv * 16;
-  const int u_lo = pu * 64, u_hi = u_lo + 64;
+  const int u = pu * kRedU + tx, v0 = pv * kRedV;
+  const int u_lo = pu * kRedU, u_hi = u_lo + kRedU;
   T acc[4] = {T(0), T(0), T(0), T(0)};
   const int4 *h = hdr + long(vloc) * ntiles;
-  for (int t0 = 0; t0 < ntiles; t0 += kThreads) {
-    // each thread tests one header; the overlapping ones are compacted in LDS
+  const long b = long(vloc) * npu * npv + blockIdx.x;
+  const int nlist = count[b];
+  const bool binned = nlist <= kBinCap;
+  const int total = binned ? nlist : ntiles;
+  for (int t0 = 0; t0 < total; t0 += kThreads) {
+    // stage up to 256 candidate tiles (the binned list, or every header)
+    // with their window in LDS, keeping those that overlap this block
     if (threadIdx.x == 0) nhit = 0;
     __syncthreads();
-    const int t = t0 + threadIdx.x;
-    if (t < ntiles) {
+    const int e = t0 + threadIdx.x;
+    if (e < total) {
+      const int t = binned ? list[b * kBinCap + e] : e;
       const int4 r = h[t];
-      if (r.z != 0 && r.x < u_hi && r.x + r.z > u_lo && r.y < v0 + 16 && r.y + r.w > v0)
-        hits[atomicAdd(&nhit, 1)] = t;
+      if (r.z != 0 && r.x < u_hi && r.x + r.z > u_lo && r.y < v0 + kRedV && r.y + r.w > v0) {
+        const int k = atomicAdd(&nhit, 1);
+        hit_t[k] = t;
+        hit_r[k] = r;
+      }
     }
     __syncthreads();
     const int n = nhit;
-    for (int e = 0; e < n; ++e) {
-      const int tt = hits[e];
-      const int4 r = h[tt];
+    for (int q0 = 0; q0 < n; ++q0) {
+      const int tt = hit_t[q0];
+      const int4 r = hit_r[q0];
       const T *src = scratch + (long(vloc) * ntiles + tt) * cap;
       const bool uin = u >= r.x && u < r.x + r.z;
 #pragma unroll
@@ -547,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
 // Backward
 // ---------------------------------------------------------------------------
 template <bool CONE, int W, int CX, int CY, typename T>
-__global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__ sino,
+__global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T *__restrict__ sino,
                                                 T *__restrict__ vol,
                                                 const int *__restrict__ seg_begin, int atomic_out,
                                                 int lds_cap, int pmax) {
@@ -575,6 +645,8 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
       const int view = P.order[seg.start + n];
       const ViewConst &g = P.views[view];
       const T *in = sino + long(view) * npix;
+      const double a12 = g.a12;
+      const T w0 = T(g.w0), w1 = T(g.w1), wsum = T(g.w0 + g.w1);
       ColLane<T> cl;
       column_table<CONE, CX, CY, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
       if (cl.bad) atomicOr(P.err, 1);
@@ -617,7 +689,7 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
             __builtin_amdgcn_wave_barrier();
             if (kin) {
               Vox<T> v;
-              voxel(P, g, c, k, v);
+              voxel(P, a12, c, k, v);
               if (v.ul < v.ur) {
                 if (v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W) {
                   T w[W];
@@ -636,14 +708,14 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
             __builtin_amdgcn_wave_barrier();
           } else if (kin) {
             Vox<T> v;
-            voxel(P, g, c, k, v);
+            voxel(P, a12, c, k, v);
             if (v.ul < v.ur) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
           }
           if (seg.t0 == seg.t1) {
-            acc0[cc] += T(g.w0 + g.w1) * contrib;
+            acc0[cc] += wsum * contrib;
           } else {
-            acc0[cc] += T(g.w0) * contrib;
-            acc1[cc] += T(g.w1) * contrib;
+            acc0[cc] += w0 * contrib;
+            acc1[cc] += w1 * contrib;
           }
         }
       }
@@ -681,7 +753,6 @@ __global__ __launch_bounds__(kThreads) void bwd(DevPlan P, const T *__restrict__
 // plan: forward one wave per workgroup, backward four waves.
 constexpr int kFwdTiles[kNumFwdTiles][2] = {{8, 4}, {4, 4}, {8, 8}};
 constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}};
-constexpr int kAxialRegionBytes = 32768;
 constexpr int kAxialPmax = 256;  // longest column profile held in LDS
 
 int axial_tiles(const DevPlan &P, int cx, int cy) {
@@ -693,11 +764,22 @@ int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu) {
   return axial_tiles(P, t[0], t[1]);
 }
 
-// Scratch of the forward reduction: one detector window per (view, tile)
-// for `chunk` views, plus one header per slot.
+// Scratch of the forward reduction for `chunk` views: one detector window
+// per (view, tile), and the header area: one header per slot, then one
+// tile count and list per (view, pixel block).
+int axial_reduce_blocks(const DevPlan &P) {
+  return ((P.nu + axial::kRedU - 1) / axial::kRedU) * ((P.nv + axial::kRedV - 1) / axial::kRedV);
+}
+
+size_t axial_hdr_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
+  const size_t slots = size_t(chunk) * size_t(axial_tiles(P, 0, tu));
+  const size_t lists = size_t(chunk) * size_t(axial_reduce_blocks(P));
+  return slots * sizeof(int4) + lists * sizeof(int) * (1 + size_t(axial::kBinCap));
+}
+
 size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
   const size_t slots = size_t(chunk) * size_t(axial_tiles(P, 0, tu));
-  return slots * size_t(tu.fwd_window_bytes) + slots * sizeof(int4);
+  return slots * size_t(tu.fwd_window_bytes) + axial_hdr_bytes(P, chunk, tu);
 }
 
 template <bool CONE, int W, int CX, int CY, typename T>
@@ -725,7 +807,7 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     const int wcap = tu.fwd_window_bytes / int(sizeof(T));
     const int lds_bytes = (wcap + pmax) * int(sizeof(T));
     const unsigned ntiles = unsigned(axial_tiles(P, 0, tu));
-    const int npt = ((P.nu + 63) / 64) * ((P.nv + 15) / 16);
+    const int npt = axial_reduce_blocks(P);
     T *volk = static_cast<T *>(volk_buf);
     hipLaunchKernelGGL(axial::to_k_fastest<T>, dim3(((P.nx + 31) / 32) * P.ny, (P.nz + 63) / 64, P.nt),
                        dim3(axial::kThreads), 0, st, static_cast<const T *>(in), volk, P.nx, P.ny,
@@ -752,13 +834,26 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
       }
       if (ev) hipEventRecord(ev[2 * npair + 1], st);
       ++npair;
+      // bins after the chunk's headers (the header area holds `chunk` views)
+      const int nb = axial_reduce_blocks(P);
+      int *count = reinterpret_cast<int *>(static_cast<int4 *>(hdr) + long(chunk) * ntiles);
+      int *list = count + long(chunk) * nb;
+      const long nc = long(ve - vb) * nb;
+      hipLaunchKernelGGL(axial::zero_ints, dim3(unsigned((nc + axial::kThreads - 1) / axial::kThreads)),
+                         dim3(axial::kThreads), 0, st, count, nc);
+      const long nh = long(ve - vb) * ntiles;
+      hipLaunchKernelGGL(axial::bin_windows,
+                         dim3(unsigned((nh + axial::kThreads - 1) / axial::kThreads)),
+                         dim3(axial::kThreads), 0, st, static_cast<const int4 *>(hdr), int(ntiles),
+                         ve - vb, (P.nu + axial::kRedU - 1) / axial::kRedU,
+                         (P.nv + axial::kRedV - 1) / axial::kRedV, count, list);
       hipLaunchKernelGGL(axial::fwd_reduce<T>, dim3(npt, ve - vb), dim3(axial::kThreads), 0, st,
                          P, static_cast<const T *>(scratch), static_cast<const int4 *>(hdr),
-                         static_cast<T *>(out), vb, int(ntiles), wcap);
+                         count, list, static_cast<T *>(out), vb, int(ntiles), wcap);
     }
   } else {
-    const int lds_cap = kAxialRegionBytes / int(sizeof(T));
-    const int lds_bytes = kAxialRegionBytes + axial::kWaves * pmax * int(sizeof(T));
+    const int lds_cap = tu.bwd_window_bytes / int(sizeof(T));
+    const int lds_bytes = tu.bwd_window_bytes + axial::kWaves * pmax * int(sizeof(T));
     dim3 grid(unsigned(axial_tiles(P, 1, tu)), nrows);
     const auto *i = static_cast<const T *>(in);
     auto *o = static_cast<T *>(out);

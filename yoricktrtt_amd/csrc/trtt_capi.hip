@@ -25,6 +25,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
                         hipStream_t st);
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu);
 size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
+size_t axial_hdr_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
 }  // namespace trtt
 
 using trtt::ViewConst;
@@ -530,10 +531,10 @@ bool alloc_axial_scratch(trtt_hip_plan *p) {
   long chunk = long(budget / per_view);
   chunk = std::max(1L, std::min(chunk, p->nviews));
   p->axial_chunk = int(chunk);
-  const size_t slots = size_t(chunk) * size_t(trtt::axial_tiles(p->dev, 0, p->tune));
-  const size_t win_bytes = per_view * size_t(chunk) - slots * sizeof(int4);
+  const size_t hdr_bytes = trtt::axial_hdr_bytes(p->dev, int(chunk), p->tune);
+  const size_t win_bytes = per_view * size_t(chunk) - hdr_bytes;
   if (!hip_ok(hipMalloc(&p->d_scratch, win_bytes), "hipMalloc scratch")) return false;
-  if (!hip_ok(hipMalloc(&p->d_hdr, slots * sizeof(int4)), "hipMalloc scratch")) return false;
+  if (!hip_ok(hipMalloc(&p->d_hdr, hdr_bytes), "hipMalloc scratch")) return false;
   // k-fastest copy of the volume series read by the forward
   if (!p->d_volk) {
     const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
@@ -714,6 +715,7 @@ int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *p, trtt_hip_tuning_t *t) {
   t->bwd_tile = p->tune.bwd_tile;
   t->fwd_window_bytes = p->tune.fwd_window_bytes;
   t->fwd_views_per_block = p->tune.fwd_views_per_block;
+  t->bwd_window_bytes = p->tune.bwd_window_bytes;
   return TRTT_SUCCESS;
 }
 
@@ -726,6 +728,8 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
     return fail("forward window must be 4096..65536 bytes, a multiple of 256");
   if (t->fwd_views_per_block < 0 || t->fwd_views_per_block > 64)
     return fail("views per block must be 0..64");
+  if (t->bwd_window_bytes < 4096 || t->bwd_window_bytes > 131072 || t->bwd_window_bytes % 256)
+    return fail("backward window must be 4096..131072 bytes, a multiple of 256");
   if (p->axial_w <= 0) return TRTT_SUCCESS;  // general kernels: nothing to tune
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return TRTT_FAILURE;
@@ -734,6 +738,7 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
   p->tune.bwd_tile = t->bwd_tile;
   p->tune.fwd_window_bytes = t->fwd_window_bytes;
   p->tune.fwd_views_per_block = t->fwd_views_per_block;
+  p->tune.bwd_window_bytes = t->bwd_window_bytes;
   if (old.fwd_tile != p->tune.fwd_tile || old.fwd_window_bytes != p->tune.fwd_window_bytes) {
     // hipFree waits for the device: no launch of this plan is in flight
     if (!alloc_axial_scratch(p)) {

@@ -69,6 +69,7 @@ struct AxialTuning {
   int bwd_tile = 0;
   int fwd_window_bytes = 16384;   // private LDS window of a forward wave
   int fwd_views_per_block = 0;    // 0: from the grid size
+  int bwd_window_bytes = 32768;   // shared LDS window of a backward workgroup
 };
 
 }  // namespace trtt
