@@ -28,7 +28,8 @@ class Tuning(C.Structure):
     """trtt_hip_tuning_t: launch shape of the axial kernels of a plan."""
 
     _fields_ = [("fwd_tile", C.c_int), ("bwd_tile", C.c_int), ("fwd_window_bytes", C.c_int),
-                ("fwd_views_per_block", C.c_int), ("bwd_window_bytes", C.c_int)]
+                ("fwd_views_per_block", C.c_int), ("bwd_window_bytes", C.c_int),
+                ("fwd_chunk_views", C.c_int)]
 
 
 class LF2DParameters(C.Structure):

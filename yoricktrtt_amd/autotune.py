@@ -28,6 +28,7 @@ BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4"}
 # detector footprint sends its voxels down the slow global-atomic path)
 WINDOWS = {0: (12288, 16384, 24576), 1: (8192, 12288, 16384), 2: (24576, 32768)}
 VPBS = (0, 1, 2, 4)
+CHUNKS = (0, 2, 4)  # views per forward scratch round (0: as many as fit the budget)
 BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384)}
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "yoricktrtt_amd", "tuning.json")
 
@@ -54,13 +55,14 @@ def set_tuning(P, **kw):
         raise RuntimeError(_lib.last_error())
 
 
-def candidates(windows=WINDOWS, vpbs=VPBS):
+def candidates(windows=WINDOWS, vpbs=VPBS, chunks=None):
     """Forward and backward candidates, tuned independently (the two
     kernels share no launch parameter).  ``windows`` maps a forward tile to
     its window sizes, or is one tuple for every tile."""
     per_tile = windows if isinstance(windows, dict) else {t: windows for t in FWD_TILES}
-    fwd = [dict(fwd_tile=t, fwd_window_bytes=wb, fwd_views_per_block=v)
-           for t in FWD_TILES for wb, v in itertools.product(per_tile[t], vpbs)]
+    chunks = CHUNKS if chunks is None else chunks
+    fwd = [dict(fwd_tile=t, fwd_window_bytes=wb, fwd_views_per_block=v, fwd_chunk_views=c)
+           for t in FWD_TILES for wb, v, c in itertools.product(per_tile[t], vpbs, chunks)]
     bwd = [dict(bwd_tile=t, bwd_window_bytes=wb) for t in BWD_TILES for wb in BWD_WINDOWS[t]]
     return fwd, bwd
 
@@ -91,7 +93,7 @@ def device_name(torch) -> str:
 
 
 def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vpbs=VPBS,
-             force=False, log=None):
+             chunks=None, force=False, log=None):
     """Tune plan P in place and return the chosen tuning with its timings.
 
     x (volume) and y (sinogram) are device tensors of the plan's shapes; by
@@ -135,7 +137,7 @@ def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vp
             ts.append(time.perf_counter() - t0)
         return sorted(ts)[len(ts) // 2]
 
-    fwd_c, bwd_c = candidates(windows, vpbs)
+    fwd_c, bwd_c = candidates(windows, vpbs, chunks)
     best_f, best_b = None, None
     rows = []
     for c in fwd_c:

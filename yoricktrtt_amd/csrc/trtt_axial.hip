@@ -800,7 +800,8 @@ template <bool CONE, int W, typename T>
 hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
                           const int *seg_begin, int nrows, int views_per_block, int atomic_out,
                           int pmax, void *scratch, void *hdr, int chunk, void *volk_buf,
-                          int plain, const AxialTuning &tu, hipEvent_t *ev, hipStream_t st) {
+                          int plain, const AxialTuning &tu, hipEvent_t *ev,
+                          const AxialOverlap *ov, int nbuf, hipStream_t st) {
   // ev (optional): pairs of events around each main kernel launch
   int npair = 0;
   if (job == 0) {
@@ -812,12 +813,19 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     hipLaunchKernelGGL(axial::to_k_fastest<T>, dim3(((P.nx + 31) / 32) * P.ny, (P.nz + 63) / 64, P.nt),
                        dim3(axial::kThreads), 0, st, static_cast<const T *>(in), volk, P.nx, P.ny,
                        P.nz);
-    for (int vb = 0; vb < P.nviews; vb += chunk) {
+    const int nchunks = (P.nviews + chunk - 1) / chunk;
+    const bool overlap = ov && ov->aux && nbuf == 2 && nchunks > 1;
+    const size_t win_elems = size_t(chunk) * ntiles * size_t(wcap);
+    const size_t hdr_bytes = axial_hdr_bytes(P, chunk, tu);
+    for (int c = 0, vb = 0; vb < P.nviews; vb += chunk, ++c) {
       const int ve = vb + chunk < P.nviews ? vb + chunk : P.nviews;
       dim3 grid(ntiles, (ve - vb + views_per_block - 1) / views_per_block);
+      const int b = overlap ? (c & 1) : 0;
       auto *o = static_cast<T *>(out);
-      auto *sc = static_cast<T *>(scratch);
-      auto *hd = static_cast<int4 *>(hdr);
+      auto *sc = static_cast<T *>(scratch) + b * win_elems;
+      auto *hd = reinterpret_cast<int4 *>(static_cast<char *>(hdr) + b * hdr_bytes);
+      // buffer b is free once the reduction of chunk c - 2 has read it
+      if (overlap && c >= 2) hipStreamWaitEvent(st, ov->red_done[b], 0);
       if (ev) hipEventRecord(ev[2 * npair], st);
       switch (tu.fwd_tile) {
         case 1:
@@ -834,22 +842,34 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
       }
       if (ev) hipEventRecord(ev[2 * npair + 1], st);
       ++npair;
+      hipStream_t rs = st;
+      if (overlap) {
+        hipEventRecord(ov->fwd_done[b], st);
+        hipStreamWaitEvent(ov->aux, ov->fwd_done[b], 0);
+        rs = ov->aux;
+      }
       // bins after the chunk's headers (the header area holds `chunk` views)
       const int nb = axial_reduce_blocks(P);
-      int *count = reinterpret_cast<int *>(static_cast<int4 *>(hdr) + long(chunk) * ntiles);
+      int *count = reinterpret_cast<int *>(hd + long(chunk) * ntiles);
       int *list = count + long(chunk) * nb;
       const long nc = long(ve - vb) * nb;
       hipLaunchKernelGGL(axial::zero_ints, dim3(unsigned((nc + axial::kThreads - 1) / axial::kThreads)),
-                         dim3(axial::kThreads), 0, st, count, nc);
+                         dim3(axial::kThreads), 0, rs, count, nc);
       const long nh = long(ve - vb) * ntiles;
       hipLaunchKernelGGL(axial::bin_windows,
                          dim3(unsigned((nh + axial::kThreads - 1) / axial::kThreads)),
-                         dim3(axial::kThreads), 0, st, static_cast<const int4 *>(hdr), int(ntiles),
+                         dim3(axial::kThreads), 0, rs, static_cast<const int4 *>(hd), int(ntiles),
                          ve - vb, (P.nu + axial::kRedU - 1) / axial::kRedU,
                          (P.nv + axial::kRedV - 1) / axial::kRedV, count, list);
-      hipLaunchKernelGGL(axial::fwd_reduce<T>, dim3(npt, ve - vb), dim3(axial::kThreads), 0, st,
-                         P, static_cast<const T *>(scratch), static_cast<const int4 *>(hdr),
-                         count, list, static_cast<T *>(out), vb, int(ntiles), wcap);
+      hipLaunchKernelGGL(axial::fwd_reduce<T>, dim3(npt, ve - vb), dim3(axial::kThreads), 0, rs,
+                         P, static_cast<const T *>(sc), static_cast<const int4 *>(hd), count, list,
+                         o, vb, int(ntiles), wcap);
+      if (overlap) hipEventRecord(ov->red_done[b], ov->aux);
+    }
+    if (overlap) {
+      // join: the launch stream resumes after both buffers' last reductions
+      hipStreamWaitEvent(st, ov->red_done[0], 0);
+      hipStreamWaitEvent(st, ov->red_done[1], 0);
     }
   } else {
     const int lds_cap = tu.bwd_window_bytes / int(sizeof(T));
@@ -880,7 +900,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
                         void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
-                        hipStream_t st) {
+                        const AxialOverlap *ov, int nbuf, hipStream_t st) {
   if (tu.fwd_tile < 0 || tu.fwd_tile >= kNumFwdTiles || tu.bwd_tile < 0 ||
       tu.bwd_tile >= kNumBwdTiles)
     return hipErrorInvalidValue;
@@ -888,7 +908,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
   if (cone == C && w == WW && f64 == (sizeof(TT) == 8))                                     \
     return launch_axial_t<C, WW, TT>(P, job, in, out, seg_begin, nrows, views_per_block,   \
                                      atomic_out, pmax, scratch, hdr, chunk, volk, plain, tu, ev, \
-                                     st);
+                                     ov, nbuf, st);
   AX_CASE(true, 9, float)
   AX_CASE(true, 13, float)
   AX_CASE(false, 9, float)

@@ -22,7 +22,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
                         void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
-                        hipStream_t st);
+                        const AxialOverlap *ov, int nbuf, hipStream_t st);
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu);
 size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
 size_t axial_hdr_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
@@ -343,6 +343,11 @@ void release(trtt_hip_plan *p) {
   hipFree(p->d_volk);
   for (hipEvent_t e : p->ev_fwd) hipEventDestroy(e);
   for (hipEvent_t e : p->ev_bwd) hipEventDestroy(e);
+  for (int b = 0; b < 2; ++b) {
+    if (p->overlap.fwd_done[b]) hipEventDestroy(p->overlap.fwd_done[b]);
+    if (p->overlap.red_done[b]) hipEventDestroy(p->overlap.red_done[b]);
+  }
+  if (p->overlap.aux) hipStreamDestroy(p->overlap.aux);
   delete p;
 }
 
@@ -529,12 +534,27 @@ bool alloc_axial_scratch(trtt_hip_plan *p) {
   const size_t per_view = trtt::axial_scratch_bytes(p->dev, 1, p->tune);
   const size_t budget = size_t(4) << 30;
   long chunk = long(budget / per_view);
+  if (p->tune.fwd_chunk_views > 0) chunk = std::min(chunk, long(p->tune.fwd_chunk_views));
   chunk = std::max(1L, std::min(chunk, p->nviews));
   p->axial_chunk = int(chunk);
   const size_t hdr_bytes = trtt::axial_hdr_bytes(p->dev, int(chunk), p->tune);
   const size_t win_bytes = per_view * size_t(chunk) - hdr_bytes;
-  if (!hip_ok(hipMalloc(&p->d_scratch, win_bytes), "hipMalloc scratch")) return false;
-  if (!hip_ok(hipMalloc(&p->d_hdr, hdr_bytes), "hipMalloc scratch")) return false;
+  // two buffers when the views take several chunks: the reduction of one
+  // chunk overlaps the projection of the next on a second stream
+  p->axial_nbuf = chunk < p->nviews ? 2 : 1;
+  const size_t nb = size_t(p->axial_nbuf);
+  if (!hip_ok(hipMalloc(&p->d_scratch, nb * win_bytes), "hipMalloc scratch")) return false;
+  if (!hip_ok(hipMalloc(&p->d_hdr, nb * hdr_bytes), "hipMalloc scratch")) return false;
+  if (p->axial_nbuf == 2 && !p->overlap.aux) {
+    if (!hip_ok(hipStreamCreateWithFlags(&p->overlap.aux, hipStreamNonBlocking), "hipStreamCreate"))
+      return false;
+    for (int b = 0; b < 2; ++b)
+      if (!hip_ok(hipEventCreateWithFlags(&p->overlap.fwd_done[b], hipEventDisableTiming),
+                  "hipEventCreate") ||
+          !hip_ok(hipEventCreateWithFlags(&p->overlap.red_done[b], hipEventDisableTiming),
+                  "hipEventCreate"))
+        return false;
+  }
   // k-fastest copy of the volume series read by the forward
   if (!p->d_volk) {
     const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
@@ -568,7 +588,8 @@ hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hip
     return trtt::launch_axial(p->kind == trtt::CB3D, p->axial_w, f64, p->dev, job, in, out,
                               p->d_seg_begin, nrows, int(vpb), nrows > 1 ? 1 : 0, p->axial_pmax,
                               p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
-                              p->axial_plain ? 1 : 0, p->tune, ev, st);
+                              p->axial_plain ? 1 : 0, p->tune, ev, &p->overlap, p->axial_nbuf,
+                              st);
   }
   if (job == 0)
     return trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, 0, in, out,
@@ -716,6 +737,7 @@ int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *p, trtt_hip_tuning_t *t) {
   t->fwd_window_bytes = p->tune.fwd_window_bytes;
   t->fwd_views_per_block = p->tune.fwd_views_per_block;
   t->bwd_window_bytes = p->tune.bwd_window_bytes;
+  t->fwd_chunk_views = p->tune.fwd_chunk_views;
   return TRTT_SUCCESS;
 }
 
@@ -730,6 +752,7 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
     return fail("views per block must be 0..64");
   if (t->bwd_window_bytes < 4096 || t->bwd_window_bytes > 131072 || t->bwd_window_bytes % 256)
     return fail("backward window must be 4096..131072 bytes, a multiple of 256");
+  if (t->fwd_chunk_views < 0) return fail("views per forward chunk must be >= 0");
   if (p->axial_w <= 0) return TRTT_SUCCESS;  // general kernels: nothing to tune
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return TRTT_FAILURE;
@@ -739,7 +762,9 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
   p->tune.fwd_window_bytes = t->fwd_window_bytes;
   p->tune.fwd_views_per_block = t->fwd_views_per_block;
   p->tune.bwd_window_bytes = t->bwd_window_bytes;
-  if (old.fwd_tile != p->tune.fwd_tile || old.fwd_window_bytes != p->tune.fwd_window_bytes) {
+  p->tune.fwd_chunk_views = t->fwd_chunk_views;
+  if (old.fwd_tile != p->tune.fwd_tile || old.fwd_window_bytes != p->tune.fwd_window_bytes ||
+      old.fwd_chunk_views != p->tune.fwd_chunk_views) {
     // hipFree waits for the device: no launch of this plan is in flight
     if (!alloc_axial_scratch(p)) {
       p->tune = old;

@@ -70,6 +70,15 @@ struct AxialTuning {
   int fwd_window_bytes = 16384;   // private LDS window of a forward wave
   int fwd_views_per_block = 0;    // 0: from the grid size
   int bwd_window_bytes = 32768;   // shared LDS window of a backward workgroup
+  int fwd_chunk_views = 0;        // views per forward scratch round, 0: fill the budget
+};
+
+// Second stream of the axial forward: the reduction of view chunk c runs on
+// `aux` while the main stream projects chunk c + 1 (two scratch buffers).
+struct AxialOverlap {
+  hipStream_t aux = nullptr;  // nullptr: reduce on the launch stream
+  hipEvent_t fwd_done[2] = {nullptr, nullptr};
+  hipEvent_t red_done[2] = {nullptr, nullptr};
 };
 
 }  // namespace trtt
@@ -112,4 +121,6 @@ struct trtt_hip_plan {
   bool timing = false;
   std::vector<hipEvent_t> ev_fwd, ev_bwd;
   int fwd_launches = 0, bwd_launches = 0;
+  int axial_nbuf = 1;           // forward scratch buffers (2: reduction overlapped)
+  trtt::AxialOverlap overlap;
 };
