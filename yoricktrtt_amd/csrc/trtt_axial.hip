@@ -32,18 +32,18 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kKT = 64;  // k per tile (one per lane)
 
-// F(x) - 1/2 for the cubic B-spline CDF: sign(x) G(min(|x|, 2)).
+// F(x) - 1/2 for the cubic B-spline CDF: sign(x) G(min(|x|, 2)) with, on
+// [0, 2], G(t) = 1/2 - (2-t)^4/24 + max(1-t, 0)^4/6 (the two pieces
+// 2t/3 - t^3/3 + t^4/8 on [0, 1] and 1/2 - (2-t)^4/24 on [1, 2] differ by
+// exactly (1-t)^4/6), so no per-piece select is needed.
 template <typename T>
 __device__ __forceinline__ T cdf3c(T x) {
-  T t = fabs(x);
-  t = t < T(2) ? t : T(2);
-  const T t2 = t * t;
-  const T g1 = t * (T(2.0 / 3.0) + t2 * (t * T(0.125) - T(1.0 / 3.0)));
+  const T t = fmin(fabs(x), T(2));
   const T s = T(2) - t;
-  const T s2 = s * s;
-  const T g2 = T(0.5) - s2 * s2 * T(1.0 / 24.0);
-  const T g = t > T(1) ? g2 : g1;
-  return x < T(0) ? -g : g;
+  const T r = fmin(fmax(T(1) - t, T(0)), T(1));  // one subtraction with output clamp
+  const T s2 = s * s, r2 = r * r;
+  const T g = fma(r2 * r2, T(1.0 / 6.0), fma(s2 * s2, T(-1.0 / 24.0), T(0.5)));
+  return copysign(g, x);
 }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -153,18 +153,18 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
   clampw_f(float(v.uc), float(T(P.hs0) * hu), P.nu, v.ul, v.ur, v.clipped);
 }
 
-// Weights of pixels [ul, ul+W) (zero beyond ur).  An unclipped window starts
-// at least 2h left of the centre, where the CDF is exactly 0.
+// Weights of pixels [ul, ul+W); entries from ur - ul on are not zeroed
+// (every caller stops at ur).  An unclipped window starts at least 2h left of
+// the centre, where the CDF is exactly 0.
 template <int W, typename T>
 __device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[W]) {
   const T x0 = T(double(v.ul) - v.uc) * v.su;
-  const int n = v.ur - v.ul;
   T prev = T(-0.5);
   if (v.clipped) prev = cdf3c<T>(x0);
 #pragma unroll
   for (int l = 0; l < W; ++l) {
     const T nx = cdf3c<T>(x0 + T(l + 1) * v.su);
-    w[l] = l < n ? nx - prev : T(0);
+    w[l] = nx - prev;
     prev = nx;
   }
 }
