@@ -27,10 +27,10 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def layer_specs(channels=(16, 32, 64)):
+def layer_specs(channels=(16, 32, 64), in_ch=1):
     """(name, cin, cout, ks, relu) of every convolution, in forward order."""
     c1, c2, c3 = channels
-    return [("enc1a", 1, c1, 3, True), ("enc1b", c1, c1, 3, True),
+    return [("enc1a", in_ch, c1, 3, True), ("enc1b", c1, c1, 3, True),
             ("enc2a", c1, c2, 3, True), ("enc2b", c2, c2, 3, True),
             ("bota", c2, c3, 3, True), ("botb", c3, c3, 3, True),
             ("dec2a", c2 + c3, c2, 3, True), ("dec2b", c2, c2, 3, True),
@@ -112,8 +112,8 @@ def upsample_bwd(dy):
 class UNetRef:
     """Forward/backward of the U-Net with parameters {name: (w, b)}."""
 
-    def __init__(self, params, channels=(16, 32, 64), bf16_emulation=False):
-        self.specs = layer_specs(channels)
+    def __init__(self, params, channels=(16, 32, 64), bf16_emulation=False, in_ch=1):
+        self.specs = layer_specs(channels, in_ch)
         self.p = params
         self.r = bf16 if bf16_emulation else (lambda a: np.asarray(a, dtype=np.float64))
         self.emulate = bf16_emulation
@@ -129,9 +129,11 @@ class UNetRef:
         return self.r(np.maximum(z, 0.0) if relu else z)
 
     def forward(self, x):
-        """x (n, d, h, w) -> (out, cache); out = x + head(x)."""
+        """x (n, d, h, w), or (n, d, h, w, c) with c input channels ->
+        (out, cache); out = x[..., 0] + head(x)."""
         r = self.r
-        a = {"x": x, "xin": r(x[..., None])}
+        xin = x[..., None] if x.ndim == 4 else x
+        a = {"x": x, "xin": r(xin)}
         a["e1a"] = self._conv("enc1a", a["xin"])
         a["e1b"] = self._conv("enc1b", a["e1a"])
         a["p1"], a["arg1"] = pool(a["e1b"])
@@ -148,7 +150,7 @@ class UNetRef:
         a["d1b"] = self._conv("dec1b", a["d1a"])
         w, b = self._w("head")
         head = conv(a["d1b"], w, b, 1)[..., 0]  # FP32 on the GPU, not rounded
-        a["out"] = x + head
+        a["out"] = (x if x.ndim == 4 else x[..., 0]) + head
         return a["out"], a
 
     def loss_and_grads(self, x, target):

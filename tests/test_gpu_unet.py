@@ -172,34 +172,37 @@ def _teacher_forced_layers(net, B, acts, U):
     assert np.array_equal(g["cat1"], np.concatenate([g["e1b"], U.upsample(g["d2b"])], -1))
 
 
-def test_network_vs_numpy():
+@pytest.mark.parametrize("in_ch", [1, 2])
+def test_network_vs_numpy(in_ch):
     """Forward layer by layer (teacher forced) and end to end; loss and every
     parameter gradient against the BF16-emulating numpy network."""
     from oracle import unet_ref as U
     from yoricktrtt_amd.unet import UNet3D
 
-    _say("network forward and gradients")
-    net = UNet3D(seed=5)
+    _say("network forward and gradients, input channels", in_ch)
+    net = UNet3D(seed=5, in_ch=in_ch)
     rng = np.random.default_rng(6)
     # a non-trivial head so the whole network carries signal
     flat = net.flat()
     lay = net.L["head"]
     flat[lay.w_off:lay.w_off + 16] = rng.normal(0, 0.3, size=16)
     net.set_flat(flat)
-    x = rng.uniform(0, 1, size=(2, 8, 8, 20)).astype(np.float32)
-    t = (x + 0.1 * rng.normal(size=x.shape)).astype(np.float32)
+    shape = (2, 8, 8, 20) if in_ch == 1 else (2, 8, 8, 20, in_ch)
+    x = rng.uniform(0, 1, size=shape).astype(np.float32)
+    x0 = x if in_ch == 1 else x[..., 0]
+    t = (x0 + 0.1 * rng.normal(size=x0.shape)).astype(np.float32)
     xd = torch.tensor(x).cuda()
     params = {name: net.layer_params(name) for name in net.L}
-    ref = U.UNetRef(params, net.channels, bf16_emulation=True)
+    ref = U.UNetRef(params, net.channels, bf16_emulation=True, in_ch=in_ch)
     out = net(xd).cpu().numpy()
     rout, acts = ref.forward(x.astype(np.float64))
-    _teacher_forced_layers(net, net._bufs[x.shape], acts, U)
+    _teacher_forced_layers(net, net._bufs[x.shape[:4]], acts, U)
     _say(f"  output rel L2 end to end {_l2(out, rout):.1e}")
     assert _l2(out, rout) < 1e-2
     loss = net.loss_and_grad(xd, torch.tensor(t).cuda())
     rloss, rg = ref.loss_and_grads(x.astype(np.float64), t.astype(np.float64))
-    _, g64 = U.UNetRef(params, net.channels).loss_and_grads(x.astype(np.float64),
-                                                            t.astype(np.float64))
+    _, g64 = U.UNetRef(params, net.channels, in_ch=in_ch).loss_and_grads(
+        x.astype(np.float64), t.astype(np.float64))
     assert abs(loss - rloss) <= 1e-2 * rloss
     worst = 0.0
     for name in net.L:

@@ -1,13 +1,15 @@
 """Learned warm start against the plain VMLM-B solve on held-out objects
-(seeds 12000 + j, never seen in training), with the committed network.
+(seeds 12000 + j, never seen in training), with the committed networks.
 
 Measured here, per object: the plain 50-iteration objective f50, the
 iterations and wall time the plain solve needs to come within 1e-3 of f50,
 and the same for the warm start (5 plain iterations, the U-Net, VMLM-B to
-the same objective).  The network must reach the target on every object in
-fewer iterations and less GPU wall time than the plain solve, and give a
-lower RMSE against the true object; a network that makes the objective worse
-must be rejected in favour of the plain iterate (the fallback)."""
+the same objective).  The default network (input: x5 and the objective
+gradient at x5) must reach the target on every object in fewer iterations,
+at most half the plain 50-iteration GPU wall time, with a lower RMSE against
+the true object; the image-only network must still gain over the plain
+solve; a network that makes the objective worse must be rejected in favour
+of the plain iterate (the fallback)."""
 import numpy as np
 import pytest
 
@@ -28,12 +30,15 @@ def rec():
     return Reconstruction(Setup())
 
 
-def test_trained_network_gains_over_plain(rec):
+@pytest.mark.parametrize("which", ["default", "image_only"])
+def test_trained_network_gains_over_plain(rec, which):
     from yoricktrtt_amd.unet import UNet3D
-    from yoricktrtt_amd.warmstart import DEFAULT_WEIGHTS, EVAL_SEED0, evaluate, summarize
+    from yoricktrtt_amd.warmstart import (DEFAULT_WEIGHTS, EVAL_SEED0, IMAGE_WEIGHTS, evaluate,
+                                          summarize)
 
-    _say("warm start vs plain on held-out objects")
-    net = UNet3D.load(DEFAULT_WEIGHTS)
+    _say("warm start vs plain on held-out objects:", which)
+    net = UNet3D.load(DEFAULT_WEIGHTS if which == "default" else IMAGE_WEIGHTS)
+    assert net.in_ch == (2 if which == "default" else 1)
     rows = evaluate(rec, net, [EVAL_SEED0 + j for j in range(4)], log=lambda m: _say(" ", m))
     s = summarize(rows)
     _say(" ", s)
@@ -42,7 +47,9 @@ def test_trained_network_gains_over_plain(rec):
         assert r["learned_reached"], r["seed"]
         assert r["learned_iters"] < r["plain_iters_to_tol"], r["seed"]
         assert r["rmse_learned"] < r["rmse_plain"], r["seed"]
-    assert s["time_ratio"] < 0.85
+    # target: the plain 50-iteration objective within 1e-3 in at most half
+    # the plain GPU wall time (default network); the image-only one gains less
+    assert s["time_ratio"] <= (0.5 if which == "default" else 0.85)
 
 
 def test_bad_network_falls_back_to_plain(rec):

@@ -3,7 +3,8 @@ libtrtt_hip.so (trtt_unet_* entry points).
 
 Network (3 levels, 16/32/64 channels by default, 3x3x3 convolutions, ReLU,
 2x2x2 max pooling, nearest upsampling, skip connections by concatenation,
-1x1x1 linear head), predicting a correction of its input:
+1x1x1 linear head), predicting a correction of its input (of input channel
+0 when it has several):
 
     out = x + head(dec1(cat(enc1, up(dec2(cat(enc2, up(bottom)))))))
 
@@ -31,10 +32,10 @@ def _pad16(k):
     return (k + 15) // 16 * 16
 
 
-def layer_specs(channels=(16, 32, 64)):
+def layer_specs(channels=(16, 32, 64), in_ch=1):
     """(name, cin, cout, ks, relu) of every convolution, in forward order."""
     c1, c2, c3 = channels
-    return [("enc1a", 1, c1, 3, True), ("enc1b", c1, c1, 3, True),
+    return [("enc1a", in_ch, c1, 3, True), ("enc1b", c1, c1, 3, True),
             ("enc2a", c1, c2, 3, True), ("enc2b", c2, c2, 3, True),
             ("bota", c2, c3, 3, True), ("botb", c3, c3, 3, True),
             ("dec2a", c2 + c3, c2, 3, True), ("dec2b", c2, c2, 3, True),
@@ -56,16 +57,18 @@ class _Layer:
 class UNet3D:
     """The warm-start network on one GPU."""
 
-    def __init__(self, channels=(16, 32, 64), seed=0, device="cuda"):
+    def __init__(self, channels=(16, 32, 64), seed=0, device="cuda", in_ch=1):
         import torch
 
         self.torch = torch
         self.lib = _lib.load()
         self.channels = tuple(int(c) for c in channels)
+        self.in_ch = int(in_ch)
+        self.meta = {}  # extra scalars saved with the weights (e.g. input scales)
         self.device = torch.device(device)
         self.layers = []
         off = 0
-        for spec in layer_specs(self.channels):
+        for spec in layer_specs(self.channels, self.in_ch):
             lay = _Layer(*spec, off)
             self.layers.append(lay)
             off += lay.size
@@ -128,13 +131,17 @@ class UNet3D:
             self._check(r, "prep_weights")
 
     def save(self, path):
-        np.savez(path, params=self.flat(), channels=np.array(self.channels))
+        np.savez(path, params=self.flat(), channels=np.array(self.channels),
+                 in_ch=np.array(self.in_ch),
+                 **{"meta_" + k: np.array(v, dtype=np.float64) for k, v in self.meta.items()})
 
     @classmethod
     def load(cls, path, device="cuda"):
         with np.load(path, allow_pickle=False) as z:
-            net = cls(tuple(int(c) for c in z["channels"]), device=device)
+            in_ch = int(z["in_ch"]) if "in_ch" in z.files else 1
+            net = cls(tuple(int(c) for c in z["channels"]), device=device, in_ch=in_ch)
             net.set_flat(z["params"])
+            net.meta = {k[5:]: float(z[k]) for k in z.files if k.startswith("meta_")}
         return net
 
     # --------------------------------------------------------------- helpers
@@ -159,7 +166,7 @@ class UNet3D:
         bf = dict(dtype=torch.bfloat16, device=self.device)
         B = {}
         sizes16 = {
-            "xin": P1, "e1a": P1 * c1, "e1b": P1 * c1, "p1": P2 * c1, "e2a": P2 * c2,
+            "xin": P1 * self.in_ch, "e1a": P1 * c1, "e1b": P1 * c1, "p1": P2 * c1, "e2a": P2 * c2,
             "e2b": P2 * c2, "p2": P3 * c2, "ba": P3 * c3, "bb": P3 * c3, "cat2": P2 * (c2 + c3),
             "d2a": P2 * c2, "d2b": P2 * c2, "cat1": P1 * (c1 + c2), "d1a": P1 * c1, "d1b": P1 * c1,
             "head16": P1, "dhead": P1,
@@ -211,17 +218,24 @@ class UNet3D:
     # --------------------------------------------------------------- forward
     def _forward(self, x, target=None, want_grad=False):
         lib, st = self.lib, self._stream()
-        if x.dim() == 3:
+        if self.in_ch == 1 and x.dim() == 3:
+            x = x.unsqueeze(0)
+        if self.in_ch > 1 and x.dim() == 4:
             x = x.unsqueeze(0)
         x = x.contiguous()
         if x.dtype != self.torch.float32:
             raise TypeError("U-Net input must be float32")
-        n, d, h, w = x.shape
+        if (self.in_ch == 1 and x.dim() != 4) or (self.in_ch > 1 and (
+                x.dim() != 5 or x.shape[-1] != self.in_ch)):
+            raise ValueError(f"expected [n][d][h][w] input with {self.in_ch} channel(s) last")
+        n, d, h, w = x.shape[:4]
         c1, c2, c3 = self.channels
         B = self._buffers(n, d, h, w)
         L1, L2, L3 = (n, d, h, w), (n, d // 2, h // 2, w // 2), (n, d // 4, h // 4, w // 4)
         P1 = n * d * h * w
-        self._check(lib.trtt_unet_cast_bf16(_vp(x), _vp(B["xin"]), P1, st), "cast")
+        self._check(lib.trtt_unet_cast_bf16(_vp(x), _vp(B["xin"]), P1 * self.in_ch, st), "cast")
+        # the network corrects channel 0
+        base = x if self.in_ch == 1 else x[..., 0].contiguous()
         self._conv("enc1a", B["xin"], B["e1a"], L1)
         self._conv("enc1b", B["e1a"], B["e1b"], L1)
         self._check(lib.trtt_unet_pool(_vp(B["e1b"]), _vp(B["p1"]), _vp(B["arg1"]), None, None, *L1,
@@ -246,17 +260,19 @@ class UNet3D:
         self._conv("dec1b", B["d1a"], B["d1b"], L1)
         self._conv("head", B["d1b"], B["head16"], L1, y32=B["head32"])
         if target is not None:
-            target = target.reshape(x.shape).contiguous()
+            target = target.reshape(base.shape).contiguous()
             self.loss_dev.zero_()
-        r = lib.trtt_unet_residual_mse(_vp(x), _vp(B["head32"]), _vp(target), _vp(B["out"]),
+        r = lib.trtt_unet_residual_mse(_vp(base), _vp(B["head32"]), _vp(target), _vp(B["out"]),
                                        _vp(B["dhead"]) if want_grad else None, P1,
                                        _vp(self.loss_dev) if target is not None else None, st)
         self._check(r, "residual")
         return B, (L1, L2, L3)
 
     def __call__(self, x):
-        """Warm start of a volume (or a batch [n][d][h][w]) in FP32."""
-        shape = x.shape
+        """Warm start of a volume (or a batch [n][d][h][w]) in FP32; with
+        several input channels, x is [n][d][h][w][c] (or [d][h][w][c]) and the
+        result has the shape of channel 0."""
+        shape = x.shape if self.in_ch == 1 else x.shape[:-1]
         B, _ = self._forward(x)
         return B["out"].reshape(shape).clone()
 

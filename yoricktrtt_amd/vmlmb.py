@@ -35,6 +35,7 @@ class Result:
     status: str
     history: list = field(default_factory=list)
     pairs: list = field(default_factory=list)  # L-BFGS memory (s, y, 1/s'y) at exit
+    grad: object = None  # gradient at x
 
 
 def _project(x, lower, upper):
@@ -151,4 +152,5 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
         if maxeval is not None and neval >= maxeval:
             status = "maxeval"
             break
-    return Result(x=x, f=f, niter=it, neval=neval, status=status, history=history, pairs=pairs)
+    return Result(x=x, f=f, niter=it, neval=neval, status=status, history=history, pairs=pairs,
+                  grad=g)

@@ -18,7 +18,9 @@ the optimiser run on the GPU (projectors, TV and U-Net kernels of
 libtrtt_hip.so).
 
 Use.  ``learned_solve`` runs 5 plain iterations, applies the network
-(``out = x5 + U(x5)``, projected on x >= 0), and keeps that start only if it
+(``out = x5 + U(x5, g5)``, projected on x >= 0, where g5 is the objective
+gradient at x5 scaled by a global factor stored with the weights; the
+image-only variant sees x5 alone), and keeps that start only if it
 lowers the objective below f(x5) -- otherwise the plain iterate is the start
 (the plain solve is the fallback); VMLM-B then continues until the objective
 reaches the plain 50-iteration value within ``tol`` relative, or the
@@ -37,7 +39,10 @@ from dataclasses import asdict, dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "warmstart_unet_cone64.npz")
+# default network: input channels (x5, objective gradient at x5)
+DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "warmstart_unet_cone64_grad.npz")
+# image-only network: input x5
+IMAGE_WEIGHTS = os.path.join(HERE, "weights", "warmstart_unet_cone64.npz")
 TRAIN_SEED0, EVAL_SEED0 = 10000, 12000
 
 
@@ -123,29 +128,47 @@ class Reconstruction:
                      pairs=pairs)
 
 
-def make_pairs(rec: Reconstruction, seeds, log=print):
-    """(x5, x50) of the plain method for every seed, on the device."""
+def make_pairs(rec: Reconstruction, seeds, log=print, grad_input=False):
+    """(x5, x50) of the plain method for every seed, on the device.  With
+    ``grad_input`` the input also holds the objective gradient at x5 as a
+    second channel (raw; scaled by :func:`grad_scale`).  The 50 iterations run
+    as 5 + 45 with the L-BFGS memory carried over, which is the same iteration
+    sequence."""
     from .phantoms import body_phantom
 
     torch, s = rec.torch, rec.s
-    X5 = torch.empty((len(seeds), s.n, s.n, s.n), dtype=torch.float32, device=rec.device)
-    X50 = torch.empty_like(X5)
+    shape = (len(seeds), s.n, s.n, s.n) + ((2,) if grad_input else ())
+    X5 = torch.empty(shape, dtype=torch.float32, device=rec.device)
+    X50 = torch.empty((len(seeds), s.n, s.n, s.n), dtype=torch.float32, device=rec.device)
     t0 = time.perf_counter()
     for i, seed in enumerate(seeds):
         rec.simulate(body_phantom(s.n, seed), seed)
-
-        def cb(it, x, f, i=i):
-            if it == s.n_first:
-                X5[i].copy_(x)
-
-        r = rec.solve(rec.zeros(), s.n_plain, callback=cb)
-        if r.niter < s.n_first:
-            X5[i].copy_(r.x)
+        r5 = rec.solve(rec.zeros(), s.n_first)
+        if grad_input:
+            X5[i, ..., 0].copy_(r5.x)
+            X5[i, ..., 1].copy_(r5.grad)
+        else:
+            X5[i].copy_(r5.x)
+        r = rec.solve(r5.x, s.n_plain - r5.niter, pairs=r5.pairs)
         X50[i].copy_(r.x)
         if (i + 1) % 50 == 0 or i + 1 == len(seeds):
             log(f"pairs {i + 1}/{len(seeds)}  {time.perf_counter() - t0:.1f} s  "
-                f"(last: {r.niter} it, {r.status})")
+                f"(last: {r5.niter + r.niter} it, {r.status})")
     return X5, X50
+
+
+def grad_scale(X5) -> float:
+    """One global factor bringing the gradient channel to unit RMS over the
+    training inputs (stored with the weights)."""
+    g = X5[..., 1]
+    return float(1.0 / max(g.pow(2).mean().sqrt().item(), 1e-30))
+
+
+def net_input(net, x5, g5):
+    """The network input for an iterate and its gradient."""
+    if net.in_ch == 1:
+        return x5
+    return net.torch.stack([x5, g5 * float(net.meta["grad_scale"])], dim=-1)
 
 
 def train(net, X5, X50, epochs, batch=4, lr=1e-3, seed=0, log=print):
@@ -195,7 +218,7 @@ def learned_solve(rec: Reconstruction, net, fstop, budget=None, keep_memory=True
     budget = s.n_plain if budget is None else budget
     t0 = _sync()
     r5 = rec.solve(rec.zeros(), s.n_first)
-    xw = net(r5.x).clamp_(min=0.0) if net is not None else None
+    xw = net(net_input(net, r5.x, r5.grad)).clamp_(min=0.0) if net is not None else None
     fw = rec.objective(xw) if xw is not None else math.inf
     use_net = fw < r5.f
     start = xw if use_net else r5.x
@@ -214,9 +237,9 @@ def evaluate(rec: Reconstruction, net, seeds, log=print, keep_memory=True):
     rows = []
     # warm-up of every launch path outside the timed solves
     rec.simulate(body_phantom(s.n, seeds[0]), seeds[0])
-    rec.solve(rec.zeros(), 3)
+    rw = rec.solve(rec.zeros(), 3)
     if net is not None:
-        net(rec.zeros())
+        net(net_input(net, rw.x, rw.grad))
     for seed in seeds:
         truth = body_phantom(s.n, seed)
         tt = torch.from_numpy(truth).to(rec.device)
@@ -273,6 +296,8 @@ def main(argv=None):
     ap.add_argument("--report", default=None, help="JSON report of the evaluation")
     ap.add_argument("--fresh-memory", action="store_true",
                     help="restart VMLM-B after the network with an empty L-BFGS memory")
+    ap.add_argument("--image-only", action="store_true",
+                    help="network input x5 only (default: x5 and the objective gradient at x5)")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -287,11 +312,18 @@ def main(argv=None):
     setup = Setup()
     rec = Reconstruction(setup)
     report = dict(setup=asdict(setup))
+    grad_input = not args.image_only
+    if args.image_only and args.weights == DEFAULT_WEIGHTS:
+        args.weights = IMAGE_WEIGHTS
     if args.mode == "train":
         t0 = time.perf_counter()
-        X5, X50 = make_pairs(rec, list(range(TRAIN_SEED0, TRAIN_SEED0 + args.n_train)), log)
+        X5, X50 = make_pairs(rec, list(range(TRAIN_SEED0, TRAIN_SEED0 + args.n_train)), log,
+                             grad_input=grad_input)
         report["pairs_time"] = time.perf_counter() - t0
-        net = UNet3D(seed=0)
+        net = UNet3D(seed=0, in_ch=2 if grad_input else 1)
+        if grad_input:
+            net.meta["grad_scale"] = grad_scale(X5)
+            X5[..., 1] *= net.meta["grad_scale"]
         t0 = time.perf_counter()
         report["train_loss"] = train(net, X5, X50, args.epochs, args.batch, args.lr, log=log)
         report["train_time"] = time.perf_counter() - t0
