@@ -183,6 +183,53 @@ __device__ __forceinline__ void vweights(const Col<T> &c, int lane, T (&cv)[W]) 
   }
 }
 
+// Row loops with a wave-uniform row count N (1..W), dispatched once per
+// column so the rows run without per-row branches.
+template <int N, typename T>
+__device__ __forceinline__ void outer_rows(T *dst, int RW, const T *cv, T pu) {
+  T t[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) t[m] = dst[m * RW];
+#pragma unroll
+  for (int m = 0; m < N; ++m) dst[m * RW] = t[m] + cv[m] * pu;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ T q_rows(const T *src, int RW, const T *cv) {
+  T q = T(0);
+#pragma unroll
+  for (int m = 0; m < N; ++m) q += cv[m] * src[m * RW];
+  return q;
+}
+
+template <int W, typename T>
+__device__ __forceinline__ void outer_n(int n, T *dst, int RW, const T (&cv)[W], T pu) {
+  switch (n) {
+#define TRTT_OUTER(NN) \
+  case NN:             \
+    if (NN <= W) outer_rows<(NN <= W ? NN : 1), T>(dst, RW, cv, pu); \
+    break;
+    TRTT_OUTER(1) TRTT_OUTER(2) TRTT_OUTER(3) TRTT_OUTER(4) TRTT_OUTER(5) TRTT_OUTER(6)
+    TRTT_OUTER(7) TRTT_OUTER(8) TRTT_OUTER(9) TRTT_OUTER(10) TRTT_OUTER(11) TRTT_OUTER(12)
+    TRTT_OUTER(13)
+#undef TRTT_OUTER
+    default: break;
+  }
+}
+
+template <int W, typename T>
+__device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) {
+  switch (n) {
+#define TRTT_Q(NN) \
+  case NN:         \
+    return NN <= W ? q_rows<(NN <= W ? NN : 1), T>(src, RW, cv) : T(0);
+    TRTT_Q(1) TRTT_Q(2) TRTT_Q(3) TRTT_Q(4) TRTT_Q(5) TRTT_Q(6) TRTT_Q(7) TRTT_Q(8) TRTT_Q(9)
+    TRTT_Q(10) TRTT_Q(11) TRTT_Q(12) TRTT_Q(13)
+#undef TRTT_Q
+    default: return T(0);
+  }
+}
+
 // Slow path for one voxel: arbitrary window sizes, direct global memory.
 template <bool FWD, typename T>
 __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T val, T *out,
@@ -449,11 +496,15 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
             // unclipped windows start on strictly monotone pixels: plain
             // accumulation; windows clamped at a detector edge may share
             // their first pixel: LDS atomics, issued after the plain steps
+            // The plain steps run over all W pixels: past the footprint the
+            // weights are exactly 0, and past a right detector edge they
+            // land beyond the column span, which is never read (the profile
+            // has W spare entries).
             const bool pl = plain && !v.clipped;
             if (pl) {
 #pragma unroll
               for (int l = 0; l < W; ++l) {
-                if (l < v.ur - v.ul) pp[l] += w[l] * val;
+                pp[l] += w[l] * val;
                 __builtin_amdgcn_wave_barrier();
               }
             }
@@ -472,16 +523,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           // the rows of one column are distinct: one batch of reads, FMAs and
           // writes per lane
           T *dst0 = win + (c.vl - vl) * RW + (lo - ul);
-          for (int u = lane; u < span; u += 64) {
-            const T pu = prof[u];
-            T *dst = dst0 + u;
-            T t[W];
-#pragma unroll
-            for (int m = 0; m < W; ++m) t[m] = m < npv ? dst[m * RW] : T(0);
-#pragma unroll
-            for (int m = 0; m < W; ++m)
-              if (m < npv) dst[m * RW] = t[m] + cv[m] * pu;
-          }
+          for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
           __builtin_amdgcn_wave_barrier();
         } else if (act) {
           T dummy = T(0);
@@ -625,7 +667,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
   static_assert(NCOL <= 64, "one column per lane");
   T *region = reinterpret_cast<T *>(g_smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  T *qrow = region + lds_cap + wave * pmax;
+  T *qrow = region + lds_cap + wave * (pmax + W);  // W zero entries past each Q row
   __shared__ int red[4 * kWaves];
   TileMap tm;
   tm.init(P, CX, CY);
@@ -678,14 +720,8 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
           T contrib = T(0);
           if (use_q) {
             const T *src0 = region + (c.vl - vl) * RW + (lo - ul);
-            for (int u = lane; u < span; u += 64) {
-              const T *src = src0 + u;
-              T qv = T(0);
-#pragma unroll
-              for (int m = 0; m < W; ++m)
-                if (m < npv) qv += cv[m] * src[m * RW];
-              qrow[u] = qv;
-            }
+            for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, RW, cv);
+            if (lane < W) qrow[span + lane] = T(0);
             __builtin_amdgcn_wave_barrier();
             if (kin) {
               Vox<T> v;
@@ -696,9 +732,10 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
                   uweights<W>(v, w);
                   const T *qq = qrow + (v.ul - lo);
                   T sum = T(0);
+                  // all W pixels: zero weights past the footprint, zero Q
+                  // past the column span (right detector edge)
 #pragma unroll
-                  for (int l = 0; l < W; ++l)
-                    if (l < v.ur - v.ul) sum += w[l] * qq[l];
+                  for (int l = 0; l < W; ++l) sum += w[l] * qq[l];
                   contrib = sum * v.rho;
                 } else {
                   slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
@@ -806,7 +843,7 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
   int npair = 0;
   if (job == 0) {
     const int wcap = tu.fwd_window_bytes / int(sizeof(T));
-    const int lds_bytes = (wcap + pmax) * int(sizeof(T));
+    const int lds_bytes = (wcap + pmax + W) * int(sizeof(T));
     const unsigned ntiles = unsigned(axial_tiles(P, 0, tu));
     const int npt = axial_reduce_blocks(P);
     T *volk = static_cast<T *>(volk_buf);
@@ -873,7 +910,7 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     }
   } else {
     const int lds_cap = tu.bwd_window_bytes / int(sizeof(T));
-    const int lds_bytes = tu.bwd_window_bytes + axial::kWaves * pmax * int(sizeof(T));
+    const int lds_bytes = tu.bwd_window_bytes + axial::kWaves * (pmax + W) * int(sizeof(T));
     dim3 grid(unsigned(axial_tiles(P, 1, tu)), nrows);
     const auto *i = static_cast<const T *>(in);
     auto *o = static_cast<T *>(out);
