@@ -476,14 +476,16 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         T val = T(0);
         Vox<T> v;
         bool act = false;
-        if (kin) {
-          voxel(P, a12, c, k, v);
-          act = v.ul < v.ur;
+        {
+          // lanes past the last slice work on a clamped k with a zero value
+          const int kc = kin ? k : P.nz - 1;
+          voxel(P, a12, c, kc, v);
+          act = kin && v.ul < v.ur;
           // volume copy with k fastest: the 64 lanes read 64 consecutive words
-          const long idx = (long(j) * P.nx + i) * P.nz + k;
+          const long idx = (long(j) * P.nx + i) * P.nz + kc;
           T x = w0 * vol0[idx];
           if (two) x += w1 * vol1[idx];
-          val = x * v.rho;
+          val = kin ? x * v.rho : T(0);
         }
         if (use_prof) {
           for (int u = lane; u < span; u += 64) prof[u] = T(0);
@@ -723,10 +725,10 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
             for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, RW, cv);
             if (lane < W) qrow[span + lane] = T(0);
             __builtin_amdgcn_wave_barrier();
-            if (kin) {
+            {
               Vox<T> v;
-              voxel(P, a12, c, k, v);
-              if (v.ul < v.ur) {
+              voxel(P, a12, c, kin ? k : P.nz - 1, v);
+              if (kin && v.ul < v.ur) {
                 if (v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W) {
                   T w[W];
                   uweights<W>(v, w);
