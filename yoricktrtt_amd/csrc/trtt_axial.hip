@@ -660,7 +660,7 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
 // ---------------------------------------------------------------------------
 // Backward
 // ---------------------------------------------------------------------------
-template <bool CONE, int W, int CX, int CY, typename T>
+template <bool CONE, int W, int CX, int CY, bool TWO, typename T>
 __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T *__restrict__ sino,
                                                 T *__restrict__ vol,
                                                 const int *__restrict__ seg_begin, int atomic_out,
@@ -681,9 +681,12 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 
   for (int s = s_begin; s < s_end; ++s) {
     const Segment seg = P.segs[s];
-    T acc0[NCOL], acc1[NCOL];
+    // TWO: some segment interpolates between two phases (second accumulator)
+    T acc0[NCOL], acc1[TWO ? NCOL : 1];
 #pragma unroll
-    for (int cc = 0; cc < NCOL; ++cc) acc0[cc] = acc1[cc] = T(0);
+    for (int cc = 0; cc < NCOL; ++cc) acc0[cc] = T(0);
+#pragma unroll
+    for (int cc = 0; cc < (TWO ? NCOL : 1); ++cc) acc1[cc] = T(0);
 
     for (int n = 0; n < seg.count; ++n) {
       const int view = P.order[seg.start + n];
@@ -750,11 +753,11 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
             voxel(P, a12, c, k, v);
             if (v.ul < v.ur) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
           }
-          if (seg.t0 == seg.t1) {
+          if (!TWO || seg.t0 == seg.t1) {
             acc0[cc] += wsum * contrib;
           } else {
             acc0[cc] += w0 * contrib;
-            acc1[cc] += w1 * contrib;
+            acc1[TWO ? cc : 0] += w1 * contrib;
           }
         }
       }
@@ -773,12 +776,12 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
         } else {
           *d0 += acc0[cc];
         }
-        if (seg.t0 != seg.t1) {
+        if (TWO && seg.t0 != seg.t1) {
           T *d1 = vol + long(seg.t1) * P.vox_per_phase + idx;
           if (atomic_out) {
-            if (acc1[cc] != T(0)) unsafeAtomicAdd(d1, acc1[cc]);
+            if (acc1[TWO ? cc : 0] != T(0)) unsafeAtomicAdd(d1, acc1[TWO ? cc : 0]);
           } else {
-            *d1 += acc1[cc];
+            *d1 += acc1[TWO ? cc : 0];
           }
         }
       }
@@ -830,9 +833,13 @@ void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, cons
 
 template <bool CONE, int W, int CX, int CY, typename T>
 void launch_bwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *in, T *out,
-                const int *seg_begin, int atomic_out, int lds_cap, int pmax) {
-  hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, T>), grid, dim3(axial::kThreads), lds_bytes, st,
-                     P, in, out, seg_begin, atomic_out, lds_cap, pmax);
+                const int *seg_begin, int atomic_out, int lds_cap, int pmax, bool two) {
+  if (two)
+    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, true, T>), grid, dim3(axial::kThreads),
+                       lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax);
+  else
+    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, false, T>), grid, dim3(axial::kThreads),
+                       lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax);
 }
 
 template <bool CONE, int W, typename T>
@@ -840,7 +847,7 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
                           const int *seg_begin, int nrows, int views_per_block, int atomic_out,
                           int pmax, void *scratch, void *hdr, int chunk, void *volk_buf,
                           int plain, const AxialTuning &tu, hipEvent_t *ev,
-                          const AxialOverlap *ov, int nbuf, hipStream_t st) {
+                          const AxialOverlap *ov, int nbuf, int two, hipStream_t st) {
   // ev (optional): pairs of events around each main kernel launch
   int npair = 0;
   if (job == 0) {
@@ -920,15 +927,15 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     switch (tu.bwd_tile) {
       case 1:
         launch_bwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
-                                     pmax);
+                                     pmax, two != 0);
         break;
       case 2:
         launch_bwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
-                                     pmax);
+                                     pmax, two != 0);
         break;
       default:
         launch_bwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
-                                     pmax);
+                                     pmax, two != 0);
     }
     if (ev) hipEventRecord(ev[1], st);
   }
@@ -939,7 +946,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
                         void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
-                        const AxialOverlap *ov, int nbuf, hipStream_t st) {
+                        const AxialOverlap *ov, int nbuf, int two, hipStream_t st) {
   if (tu.fwd_tile < 0 || tu.fwd_tile >= kNumFwdTiles || tu.bwd_tile < 0 ||
       tu.bwd_tile >= kNumBwdTiles)
     return hipErrorInvalidValue;
@@ -947,7 +954,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
   if (cone == C && w == WW && f64 == (sizeof(TT) == 8))                                     \
     return launch_axial_t<C, WW, TT>(P, job, in, out, seg_begin, nrows, views_per_block,   \
                                      atomic_out, pmax, scratch, hdr, chunk, volk, plain, tu, ev, \
-                                     ov, nbuf, st);
+                                     ov, nbuf, two, st);
   AX_CASE(true, 9, float)
   AX_CASE(true, 13, float)
   AX_CASE(false, 9, float)

@@ -22,7 +22,7 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
                         void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
-                        const AxialOverlap *ov, int nbuf, hipStream_t st);
+                        const AxialOverlap *ov, int nbuf, int two, hipStream_t st);
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu);
 size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
 size_t axial_hdr_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
@@ -266,6 +266,8 @@ void build_segments(trtt_hip_plan *p) {
     }
   }
   p->h_seg_begin[nrows] = int(p->h_segs.size());
+  p->two_phase_segs = false;
+  for (const trtt::Segment &sg : p->h_segs) p->two_phase_segs |= sg.t0 != sg.t1;
 }
 
 template <typename T>
@@ -589,7 +591,7 @@ hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hip
                               p->d_seg_begin, nrows, int(vpb), nrows > 1 ? 1 : 0, p->axial_pmax,
                               p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
                               p->axial_plain ? 1 : 0, p->tune, ev, &p->overlap, p->axial_nbuf,
-                              st);
+                              p->two_phase_segs ? 1 : 0, st);
   }
   if (job == 0)
     return trtt::launch_projector(p->kind, p->table, p->quirk_last_bin, f64, p->dev, 0, in, out,

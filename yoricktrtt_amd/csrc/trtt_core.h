@@ -122,5 +122,6 @@ struct trtt_hip_plan {
   std::vector<hipEvent_t> ev_fwd, ev_bwd;
   int fwd_launches = 0, bwd_launches = 0;
   int axial_nbuf = 1;           // forward scratch buffers (2: reduction overlapped)
+  bool two_phase_segs = false;  // some back-projection segment has t0 != t1
   trtt::AxialOverlap overlap;
 };
