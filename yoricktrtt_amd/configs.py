@@ -73,12 +73,31 @@ def _projection_times(P, x, sino):
     return fwd, bwd
 
 
+TUNE_CACHE = None  # set by main(); None: the table shipped with the package
+
+
+def _tuned(P):
+    """Apply the autotuned launch shape to an axial plan (cached per GPU and
+    geometry; tuned on the spot when missing)."""
+    import os
+
+    from .autotune import autotune
+
+    if P.path > 0:
+        cache = TUNE_CACHE or os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning",
+                                           "tuning.json")
+        autotune(P, cache=cache)
+    return P
+
+
 def _solve(obj, x0, iters, truth=None, sample=None):
-    """VMLM-B with positivity; per-iteration wall time and final RMSE of the
-    B-spline samples against the truth."""
+    """VMLM-B with positivity; per-iteration wall time (after one warm-up
+    evaluation of the objective) and final RMSE of the B-spline samples
+    against the truth."""
     from .vmlmb import vmlmb
 
     torch = _torch()
+    obj(x0.clone(), torch.empty_like(x0))  # first-call costs outside the timing
     t0 = _sync()
     r = vmlmb(obj, x0, lower=0.0, maxiter=iters)
     t = _sync() - t0
@@ -219,7 +238,7 @@ def config3(iters=30, tv_weight=5.0, tv_eps=0.05, photons=1e5, mu_scale=0.02):
     geo = cone_setup(n)
     dev = torch.device("cuda")
     truth = torch.from_numpy(ellipsoids_3d((n, n, n), 30, seed=3)).to(dev)
-    P = Projector(geo, model="cubic", precision="f32")
+    P = _tuned(Projector(geo, model="cubic", precision="f32"))
     sino = torch.zeros(P.sino_shape, dtype=torch.float32, device=dev)
     fwd, bwd = _projection_times(P, truth, sino)
     sino.zero_()
@@ -287,8 +306,8 @@ def config4(iters=30, tv_weight=5.0, tv_eps=0.05, wt=1.0, photons=1e5, mu_scale=
     geo, phase, counts = config4_setup(nt)
     dev = torch.device("cuda")
     truth = torch.from_numpy(respiratory_series(nt)).to(dev)
-    P = Projector(geo, model="cubic", precision="f32", nt=nt, t0=phase, t1=phase,
-                  w0=np.ones(phase.size), w1=np.zeros(phase.size))
+    P = _tuned(Projector(geo, model="cubic", precision="f32", nt=nt, t0=phase, t1=phase,
+                         w0=np.ones(phase.size), w1=np.zeros(phase.size)))
     sino = torch.zeros(P.sino_shape, dtype=torch.float32, device=dev)
     fwd, bwd = _projection_times(P, truth, sino)
     sino.zero_()
@@ -319,7 +338,10 @@ def main(argv=None):
     ap.add_argument("configs", nargs="*", type=int, default=[1, 2, 3, 4])
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--out", default=None, help="append JSON lines here")
+    ap.add_argument("--tune-cache", default=None, help="autotuner table (default: shipped one)")
     args = ap.parse_args(argv)
+    global TUNE_CACHE
+    TUNE_CACHE = args.tune_cache
     for k in args.configs:
         kw = {} if args.iters is None else {"iters": args.iters}
         t0 = time.perf_counter()
