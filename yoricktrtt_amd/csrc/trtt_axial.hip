@@ -12,8 +12,8 @@
 //  * Forward: the lanes accumulate the column's axial profile
 //    P[u] = sum_k x_k rho_k w_k(u) in wave-private LDS (distinct addresses
 //    per step, no collisions), then add the outer product P x cv into the
-//    workgroup's detector window in LDS; the window is flushed to HBM with
-//    one atomic per non-zero pixel.
+//    wave's detector window in LDS; the window goes to a scratch slot and a
+//    reduction kernel sums the slots over each detector block.
 //  * Backward: the window is staged from HBM; the lanes first form
 //    Q[u] = sum_m cv[m] y[vl+m][u] for the column, then each lane gathers its
 //    voxel's axial weights against Q.
@@ -128,16 +128,19 @@ template <typename T>
 struct Vox {
   double uc;
   T su, rho;
+  float hs;      // half support in pixels
   int ul, ur;
+  int ux;        // right bound before clamping at the detector edge
   bool clipped;  // left bound clamped at the detector edge
 };
 
-__device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int &hi,
+__device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int &hi, int &hx,
                                          bool &clipped) {
   const float l = c - hs, r = c + hs;
   const float nm1 = float(n - 1);
   lo = l < 0.0f ? 0 : (l <= nm1 ? int(floorf(l)) : n - 1);
-  hi = r < 0.0f ? 0 : (r <= nm1 ? int(floorf(r)) + 1 : n);
+  hx = int(floorf(fminf(fmaxf(r, -1.0f), 1.0e7f))) + 1;
+  hi = r < 0.0f ? 0 : min(hx, n);
   clipped = !(l >= 0.0f && l <= nm1);
 }
 
@@ -150,23 +153,45 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
   const T hu = c.hu0 * du;
   v.su = frcp(hu);
   v.rho = c.rho * du;
-  clampw_f(float(v.uc), float(T(P.hs0) * hu), P.nu, v.ul, v.ur, v.clipped);
+  v.hs = float(T(P.hs0) * hu);
+  clampw_f(float(v.uc), v.hs, P.nu, v.ul, v.ur, v.ux, v.clipped);
 }
 
-// Weights of pixels [ul, ul+W); entries from ur - ul on are not zeroed
-// (every caller stops at ur).  An unclipped window starts at least 2h left of
-// the centre, where the CDF is exactly 0.
-template <int W, typename T>
-__device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[W]) {
+// Weights of pixels [ul, ul+NP) for a voxel whose unclamped window
+// [ul, ux) has at most NP pixels: NP - 1 CDF evaluations, and the last
+// weight is 1/2 - F because the CDF is exactly 1/2 from the footprint's right
+// end on.  An unclipped window starts at least 2h left of the centre, where
+// the CDF is exactly -1/2.  Entries past the window are zero.
+template <int NP, typename T>
+__device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[NP]) {
   const T x0 = T(double(v.ul) - v.uc) * v.su;
   T prev = T(-0.5);
   if (v.clipped) prev = cdf3c<T>(x0);
 #pragma unroll
-  for (int l = 0; l < W; ++l) {
+  for (int l = 0; l + 1 < NP; ++l) {
     const T nx = cdf3c<T>(x0 + T(l + 1) * v.su);
     w[l] = nx - prev;
     prev = nx;
   }
+  w[NP - 1] = T(0.5) - prev;
+}
+
+// The weights of uweights<np> are exact on the detector when the unclamped
+// window fits np pixels, or when pixel ul + np - 1 lies past the right
+// detector edge (its weight is then never used; voxels wholly past that
+// edge have the one-pixel window [nu - 1, nu)).
+template <typename T>
+__device__ __forceinline__ bool fits(const Vox<T> &v, int np, int nu) {
+  return v.ux - v.ul <= np || v.ul + np > nu;
+}
+
+// Pixel bound of a column: every voxel window of the column has at most
+// ceil(2 hs_max) + 1 pixels, hs_max the larger half support of the two ends
+// of the k range (hs is convex in k: uc is affine in k and the distortion
+// factor sqrt(1 + uca^2/a) convex in uc).  The margin covers the rounding of
+// the window bounds; voxels that still exceed it take the slow path.
+__device__ __forceinline__ int pixel_bound(float hs_a, float hs_b) {
+  return int(ceilf(2.0f * fmaxf(hs_a, hs_b) + 1.0e-3f)) + 1;
 }
 
 // v weights of the column, broadcast from lanes.
@@ -202,6 +227,45 @@ __device__ __forceinline__ T q_rows(const T *src, int RW, const T *cv) {
   return q;
 }
 
+// Both 64-pixel halves of a column span of at most 128 pixels in one batch:
+// every read is in flight before the first write (one LDS round trip per
+// column).  All 64 lanes take part with the window's row stride; lanes past
+// the span add exactly 0 (p = 0) and so write back the value they read in
+// this batch.  Such a write can only land on a pixel of this column by
+// wrapping past the end of a window row, i.e. on the pixel of a later row,
+// whose real update comes later in program order (the DS operations of a
+// wave execute in order): every pixel ends with its real update.  The
+// highest index touched is RW * RH + 127 <= wcap + 127.
+template <int N, typename T>
+__device__ __forceinline__ void outer_rows2(T *dst, int RW, const T *cv, T p0, T p1) {
+  T t0[N], t1[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    t0[m] = dst[m * RW];
+    t1[m] = dst[m * RW + 64];
+  }
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    dst[m * RW] = t0[m] + cv[m] * p0;
+    dst[m * RW + 64] = t1[m] + cv[m] * p1;
+  }
+}
+
+template <int W, typename T>
+__device__ __forceinline__ void outer_n2(int n, T *dst, int RW, const T (&cv)[W], T p0, T p1) {
+  switch (n) {
+#define TRTT_OUTER2(NN) \
+  case NN:              \
+    if (NN <= W) outer_rows2<(NN <= W ? NN : 1), T>(dst, RW, cv, p0, p1); \
+    break;
+    TRTT_OUTER2(1) TRTT_OUTER2(2) TRTT_OUTER2(3) TRTT_OUTER2(4) TRTT_OUTER2(5) TRTT_OUTER2(6)
+    TRTT_OUTER2(7) TRTT_OUTER2(8) TRTT_OUTER2(9) TRTT_OUTER2(10) TRTT_OUTER2(11)
+    TRTT_OUTER2(12) TRTT_OUTER2(13)
+#undef TRTT_OUTER2
+    default: break;
+  }
+}
+
 template <int W, typename T>
 __device__ __forceinline__ void outer_n(int n, T *dst, int RW, const T (&cv)[W], T pu) {
   switch (n) {
@@ -226,6 +290,73 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
     TRTT_Q(1) TRTT_Q(2) TRTT_Q(3) TRTT_Q(4) TRTT_Q(5) TRTT_Q(6) TRTT_Q(7) TRTT_Q(8) TRTT_Q(9)
     TRTT_Q(10) TRTT_Q(11) TRTT_Q(12) TRTT_Q(13)
 #undef TRTT_Q
+    default: return T(0);
+  }
+}
+
+// Forward profile steps of one voxel whose window has at most NP pixels
+// (pp: the profile at the window's first pixel).  Unclipped windows start on
+// strictly monotone pixels across the lanes: plain accumulation over all NP
+// pixels (zero weights past the footprint; past a right detector edge they
+// land beyond the column span, which is never read, and the profile has
+// spare entries for them).  Windows clamped at the left detector edge may
+// share their first pixel: LDS atomics, issued after the plain steps.
+template <int NP, typename T>
+__device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, bool pl) {
+  T w[NP];
+  uweights<NP>(v, w);
+  if (pl) {
+#pragma unroll
+    for (int l = 0; l < NP; ++l) {
+      pp[l] += w[l] * val;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (!pl) {
+#pragma unroll
+    for (int l = 0; l < NP; ++l)
+      if (l < v.ur - v.ul) atomicAdd(pp + l, w[l] * val);
+  }
+}
+
+// Backward gather of one voxel whose window has at most NP pixels against
+// the column's Q row (qq: Q at the window's first pixel; Q is zero past the
+// column span).
+template <int NP, typename T>
+__device__ __forceinline__ T gather_steps(const Vox<T> &v, const T *qq) {
+  T w[NP];
+  uweights<NP>(v, w);
+  T sum = T(0);
+#pragma unroll
+  for (int l = 0; l < NP; ++l) sum += w[l] * qq[l];
+  return sum;
+}
+
+// Dispatch on the column's wave-uniform pixel bound np (1..W).
+template <int W, typename T>
+__device__ __forceinline__ void profile_n(int np, const Vox<T> &v, T *pp, T val, bool pl) {
+  switch (np) {
+#define TRTT_PS(NN) \
+  case NN:          \
+    if (NN <= W) profile_steps<(NN <= W ? NN : 1), T>(v, pp, val, pl); \
+    break;
+    TRTT_PS(1) TRTT_PS(2) TRTT_PS(3) TRTT_PS(4) TRTT_PS(5) TRTT_PS(6) TRTT_PS(7) TRTT_PS(8)
+    TRTT_PS(9) TRTT_PS(10) TRTT_PS(11) TRTT_PS(12) TRTT_PS(13)
+#undef TRTT_PS
+    default: break;
+  }
+}
+
+template <int W, typename T>
+__device__ __forceinline__ T gather_n(int np, const Vox<T> &v, const T *qq) {
+  switch (np) {
+#define TRTT_GS(NN) \
+  case NN:          \
+    return NN <= W ? gather_steps<(NN <= W ? NN : 1), T>(v, qq) : T(0);
+    TRTT_GS(1) TRTT_GS(2) TRTT_GS(3) TRTT_GS(4) TRTT_GS(5) TRTT_GS(6) TRTT_GS(7) TRTT_GS(8)
+    TRTT_GS(9) TRTT_GS(10) TRTT_GS(11) TRTT_GS(12) TRTT_GS(13)
+#undef TRTT_GS
     default: return T(0);
   }
 }
@@ -296,15 +427,17 @@ __device__ __forceinline__ void block_union(int &a, int &b, int &c, int &d, int 
 
 // Column hull of the u windows over the k range [kf, kl] (uc is affine in k
 // and the window bounds are monotone in uc): windows at the two ends, padded
-// by one pixel.  Voxels are still checked against it.
+// by one pixel.  Voxels are still checked against it.  np: pixel bound of
+// the column's voxel windows.
 template <typename T>
 __device__ __forceinline__ void column_hull(const DevPlan &P, const ViewConst &g, const Col<T> &c,
-                                            int kf, int kl, int &lo, int &hi) {
+                                            int kf, int kl, int &lo, int &hi, int &np) {
   Vox<T> a, b;
   voxel(P, g.a12, c, kf, a);
   voxel(P, g.a12, c, kl, b);
   lo = max(0, min(a.ul, b.ul) - 1);
   hi = min(P.nu, max(a.ur, b.ur) + 1);
+  np = pixel_bound(a.hs, b.hs);
 }
 
 // ---------------------------------------------------------------------------
@@ -316,6 +449,7 @@ template <typename T>
 struct ColLane {
   Col<T> c;
   int lo, hi;   // u hull of the column over [kf, kl]
+  int np;       // pixel bound of its voxel windows
   int i, j;     // voxel column
   bool live;    // inside the volume, wk != 0, non-empty windows
   bool bad;     // wk == 0 (voxel in the source plane)
@@ -330,11 +464,12 @@ __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &
   cl.live = false;
   cl.bad = false;
   cl.lo = cl.hi = 0;
+  cl.np = 0;
   if (lane < NCOL && cl.i < P.nx && cl.j < P.ny) {
     column<CONE>(P, g, cl.i, cl.j, cl.c);
     cl.bad = !cl.c.ok;
     if (cl.c.ok && cl.c.vl < cl.c.vr) {
-      column_hull(P, g, cl.c, kf, kl, cl.lo, cl.hi);
+      column_hull(P, g, cl.c, kf, kl, cl.lo, cl.hi, cl.np);
       cl.live = cl.lo < cl.hi;
     }
   }
@@ -344,7 +479,7 @@ __device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgc
 
 template <typename T>
 __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> &c, int &lo, int &hi,
-                                          int &i, int &j) {
+                                          int &np, int &i, int &j) {
   c.vc = bcast(src.c.vc, cc);
   c.uc0 = bcast(src.c.uc0, cc);
   c.duc = bcast(src.c.duc, cc);
@@ -357,6 +492,7 @@ __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> 
   c.ok = true;
   lo = bcast_i(src.lo, cc);
   hi = bcast_i(src.hi, cc);
+  np = bcast_i(src.np, cc);
   i = bcast_i(src.i, cc);
   j = bcast_i(src.j, cc);
 }
@@ -450,10 +586,11 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     ColLane<T> cl;
     column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
     if (cl.bad) atomicOr(P.err, 1);
-    const int ul = wred_min(cl.live ? cl.lo : INT_MAX);
-    const int ur = wred_max(cl.live ? cl.hi : INT_MIN);
-    const int vl = wred_min(cl.live ? cl.c.vl : INT_MAX);
-    const int vr = wred_max(cl.live ? cl.c.vr : INT_MIN);
+    // wave-uniform (readfirstlane: kept in scalar registers)
+    const int ul = bcast_i(wred_min(cl.live ? cl.lo : INT_MAX), 0);
+    const int ur = bcast_i(wred_max(cl.live ? cl.hi : INT_MIN), 0);
+    const int vl = bcast_i(wred_min(cl.live ? cl.c.vl : INT_MAX), 0);
+    const int vr = bcast_i(wred_max(cl.live ? cl.c.vr : INT_MIN), 0);
     const bool any = ur > ul && vr > vl;
     const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && RW * RH <= wcap;
@@ -466,8 +603,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       for (int cc = 0; cc < NCOL; ++cc) {
         if (!((live_mask >> cc) & 1ull)) continue;
         Col<T> c;
-        int lo, hi, i, j;
-        bcast_col(cl, cc, c, lo, hi, i, j);
+        int lo, hi, np, i, j;
+        bcast_col(cl, cc, c, lo, hi, np, i, j);
+        np = min(np, W);
         const int span = hi - lo;
         const int npv = c.vr - c.vl;
         const bool use_prof = in_lds && span <= pmax && npv <= W;
@@ -490,32 +628,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         if (use_prof) {
           for (int u = lane; u < span; u += 64) prof[u] = T(0);
           __builtin_amdgcn_wave_barrier();
-          const bool fast = act && v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W;
+          const bool fast = act && v.ul >= lo && v.ur <= hi && fits(v, np, P.nu);
           if (fast) {
-            T w[W];
-            uweights<W>(v, w);
-            T *pp = prof + (v.ul - lo);
-            // unclipped windows start on strictly monotone pixels: plain
-            // accumulation; windows clamped at a detector edge may share
-            // their first pixel: LDS atomics, issued after the plain steps
-            // The plain steps run over all W pixels: past the footprint the
-            // weights are exactly 0, and past a right detector edge they
-            // land beyond the column span, which is never read (the profile
-            // has W spare entries).
-            const bool pl = plain && !v.clipped;
-            if (pl) {
-#pragma unroll
-              for (int l = 0; l < W; ++l) {
-                pp[l] += w[l] * val;
-                __builtin_amdgcn_wave_barrier();
-              }
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (!pl) {
-#pragma unroll
-              for (int l = 0; l < W; ++l)
-                if (l < v.ur - v.ul) atomicAdd(pp + l, w[l] * val);
-            }
+            profile_n<W>(np, v, prof + (v.ul - lo), val, plain && !v.clipped);
           } else if (act) {
             T dummy = T(0);
             slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
@@ -525,7 +640,15 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           // the rows of one column are distinct: one batch of reads, FMAs and
           // writes per lane
           T *dst0 = win + (c.vl - vl) * RW + (lo - ul);
-          for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
+          if (span <= 128) {
+            // profile entries past the span may hold weights of pixels past
+            // the detector edge: masked to 0
+            const T p0 = lane < span ? prof[lane] : T(0);
+            const T p1 = lane + 64 < span ? prof[lane + 64] : T(0);
+            outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
+          } else {
+            for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
+          }
           __builtin_amdgcn_wave_barrier();
         } else if (act) {
           T dummy = T(0);
@@ -715,8 +838,9 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
         for (int cc = 0; cc < NCOL; ++cc) {
           if (!((live_mask >> cc) & 1ull)) continue;
           Col<T> c;
-          int lo, hi, i, j;
-          bcast_col(cl, cc, c, lo, hi, i, j);
+          int lo, hi, np, i, j;
+          bcast_col(cl, cc, c, lo, hi, np, i, j);
+          np = min(np, W);
           const int span = hi - lo;
           const int npv = c.vr - c.vl;
           const bool use_q = in_lds && span <= pmax && npv <= W;
@@ -732,16 +856,10 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               Vox<T> v;
               voxel(P, a12, c, kin ? k : P.nz - 1, v);
               if (kin && v.ul < v.ur) {
-                if (v.ul >= lo && v.ur <= hi && v.ur - v.ul <= W) {
-                  T w[W];
-                  uweights<W>(v, w);
-                  const T *qq = qrow + (v.ul - lo);
-                  T sum = T(0);
-                  // all W pixels: zero weights past the footprint, zero Q
+                if (v.ul >= lo && v.ur <= hi && fits(v, np, P.nu)) {
+                  // all np pixels: zero weights past the footprint, zero Q
                   // past the column span (right detector edge)
-#pragma unroll
-                  for (int l = 0; l < W; ++l) sum += w[l] * qq[l];
-                  contrib = sum * v.rho;
+                  contrib = gather_n<W>(np, v, qrow + (v.ul - lo)) * v.rho;
                 } else {
                   slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
                 }
@@ -852,7 +970,9 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
   int npair = 0;
   if (job == 0) {
     const int wcap = tu.fwd_window_bytes / int(sizeof(T));
-    const int lds_bytes = (wcap + pmax + W) * int(sizeof(T));
+    // window, profile with W spare entries; at least 128 entries past the
+    // window for the batched outer product of spans up to 128 pixels
+    const int lds_bytes = (wcap + max(pmax + W, 128)) * int(sizeof(T));
     const unsigned ntiles = unsigned(axial_tiles(P, 0, tu));
     const int npt = axial_reduce_blocks(P);
     T *volk = static_cast<T *>(volk_buf);
