@@ -227,6 +227,40 @@ __device__ __forceinline__ T q_rows(const T *src, int RW, const T *cv) {
   return q;
 }
 
+// Q of both 64-pixel halves of a column span of at most 128 pixels with all
+// reads in one batch.
+template <int N, typename T>
+__device__ __forceinline__ void q_rows2(const T *src, int RW, const T *cv, T &q0, T &q1) {
+  T t0[N], t1[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    t0[m] = src[m * RW];
+    t1[m] = src[m * RW + 64];
+  }
+  q0 = T(0);
+  q1 = T(0);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    q0 += cv[m] * t0[m];
+    q1 += cv[m] * t1[m];
+  }
+}
+
+template <int W, typename T>
+__device__ __forceinline__ void q_n2(int n, const T *src, int RW, const T (&cv)[W], T &q0, T &q1) {
+  q0 = q1 = T(0);
+  switch (n) {
+#define TRTT_Q2(NN) \
+  case NN:          \
+    if (NN <= W) q_rows2<(NN <= W ? NN : 1), T>(src, RW, cv, q0, q1); \
+    break;
+    TRTT_Q2(1) TRTT_Q2(2) TRTT_Q2(3) TRTT_Q2(4) TRTT_Q2(5) TRTT_Q2(6) TRTT_Q2(7) TRTT_Q2(8)
+    TRTT_Q2(9) TRTT_Q2(10) TRTT_Q2(11) TRTT_Q2(12) TRTT_Q2(13)
+#undef TRTT_Q2
+    default: break;
+  }
+}
+
 // Both 64-pixel halves of a column span of at most 128 pixels in one batch:
 // every read is in flight before the first write (one LDS round trip per
 // column).  All 64 lanes take part with the window's row stride; lanes past
@@ -822,6 +856,11 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       if (cl.bad) atomicOr(P.err, 1);
       int ul, ur, vl, vr;
       block_window(cl, red, ul, ur, vl, vr);
+      // wave-uniform (readfirstlane: kept in scalar registers)
+      ul = bcast_i(ul, 0);
+      ur = bcast_i(ur, 0);
+      vl = bcast_i(vl, 0);
+      vr = bcast_i(vr, 0);
       const bool any = ur > ul && vr > vl;
       const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
       const bool in_lds = any && RW * RH <= lds_cap;
@@ -849,7 +888,17 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
           T contrib = T(0);
           if (use_q) {
             const T *src0 = region + (c.vl - vl) * RW + (lo - ul);
-            for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, RW, cv);
+            if (span <= 128) {
+              // both halves in one batch of reads; lanes past the span read
+              // other entries of the LDS region (index < lds_cap + 128) and
+              // store zeros into the slack
+              T q0, q1;
+              q_n2<W>(npv, src0 + lane, RW, cv, q0, q1);
+              if (lane < span + W) qrow[lane] = lane < span ? q0 : T(0);
+              if (lane + 64 < span + W) qrow[lane + 64] = lane + 64 < span ? q1 : T(0);
+            } else {
+              for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, RW, cv);
+            }
             if (lane < W) qrow[span + lane] = T(0);
             __builtin_amdgcn_wave_barrier();
             {
@@ -1039,7 +1088,10 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     }
   } else {
     const int lds_cap = tu.bwd_window_bytes / int(sizeof(T));
-    const int lds_bytes = tu.bwd_window_bytes + axial::kWaves * (pmax + W) * int(sizeof(T));
+    // window, one Q row with W zero entries per wave; at least 128 entries
+    // past the window for the batched Q of spans up to 128 pixels
+    const int lds_bytes =
+        tu.bwd_window_bytes + max(axial::kWaves * (pmax + W), 128) * int(sizeof(T));
     dim3 grid(unsigned(axial_tiles(P, 1, tu)), nrows);
     const auto *i = static_cast<const T *>(in);
     auto *o = static_cast<T *>(out);
