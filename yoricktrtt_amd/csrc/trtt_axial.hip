@@ -633,9 +633,25 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     __builtin_amdgcn_wave_barrier();
 
     if (any) {
-      const unsigned long long live_mask = __ballot(cl.live);
-      for (int cc = 0; cc < NCOL; ++cc) {
-        if (!((live_mask >> cc) & 1ull)) continue;
+      // lanes past the last slice work on a clamped k with a zero value
+      const int kc = kin ? k : P.nz - 1;
+      // the voxels of the next live column are loaded while the current one
+      // is projected (volume copy with k fastest: the 64 lanes read 64
+      // consecutive words)
+      unsigned long long pend = __ballot(cl.live);
+      T xa = T(0), xb = T(0);
+      auto load_col = [&](int col) {
+        const long idx = (long(bcast_i(cl.j, col)) * P.nx + bcast_i(cl.i, col)) * P.nz + kc;
+        xa = vol0[idx];
+        xb = two ? vol1[idx] : T(0);
+      };
+      if (pend) load_col(__builtin_ctzll(pend));
+      while (pend) {
+        const int cc = __builtin_ctzll(pend);
+        pend &= pend - 1;
+        T x = w0 * xa;
+        if (two) x += w1 * xb;
+        if (pend) load_col(__builtin_ctzll(pend));
         Col<T> c;
         int lo, hi, np, i, j;
         bcast_col(cl, cc, c, lo, hi, np, i, j);
@@ -648,17 +664,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         T val = T(0);
         Vox<T> v;
         bool act = false;
-        {
-          // lanes past the last slice work on a clamped k with a zero value
-          const int kc = kin ? k : P.nz - 1;
-          voxel(P, a12, c, kc, v);
-          act = kin && v.ul < v.ur;
-          // volume copy with k fastest: the 64 lanes read 64 consecutive words
-          const long idx = (long(j) * P.nx + i) * P.nz + kc;
-          T x = w0 * vol0[idx];
-          if (two) x += w1 * vol1[idx];
-          val = kin ? x * v.rho : T(0);
-        }
+        voxel(P, a12, c, kc, v);
+        act = kin && v.ul < v.ur;
+        val = kin ? x * v.rho : T(0);
         if (use_prof) {
           for (int u = lane; u < span; u += 64) prof[u] = T(0);
           __builtin_amdgcn_wave_barrier();
