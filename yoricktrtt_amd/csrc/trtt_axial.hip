@@ -35,15 +35,20 @@ constexpr int kKT = 64;  // k per tile (one per lane)
 // F(x) - 1/2 for the cubic B-spline CDF: sign(x) G(min(|x|, 2)) with, on
 // [0, 2], G(t) = 1/2 - (2-t)^4/24 + max(1-t, 0)^4/6 (the two pieces
 // 2t/3 - t^3/3 + t^4/8 on [0, 1] and 1/2 - (2-t)^4/24 on [1, 2] differ by
-// exactly (1-t)^4/6), so no per-piece select is needed.
+// exactly (1-t)^4/6), so no per-piece select is needed.  The 1/24 and 1/6
+// are folded into the bases, s = (2-t)/24^(1/4) and r = max(1-t, 0)/6^(1/4),
+// so G = 1/2 - (s^2)^2 + (r^2)^2 is two FMAs on the squares: 8 instructions
+// with the clamp and the sign transfer.
+constexpr double kC24 = 0.45180100180492244;  // 24^(-1/4)
+constexpr double kC6 = 0.6389431042462724;    // 6^(-1/4)
+
 template <typename T>
 __device__ __forceinline__ T cdf3c(T x) {
   const T t = fmin(fabs(x), T(2));
-  const T s = T(2) - t;
-  const T r = fmin(fmax(T(1) - t, T(0)), T(1));  // one subtraction with output clamp
+  const T s = fma(t, T(-kC24), T(2.0 * kC24));
+  const T r = fmin(fmax(fma(t, T(-kC6), T(kC6)), T(0)), T(1));  // output clamp
   const T s2 = s * s, r2 = r * r;
-  const T g = fma(r2 * r2, T(1.0 / 6.0), fma(s2 * s2, T(-1.0 / 24.0), T(0.5)));
-  return copysign(g, x);
+  return copysign(fma(r2, r2, fma(s2, -s2, T(0.5))), x);
 }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -194,18 +199,25 @@ __device__ __forceinline__ int pixel_bound(float hs_a, float hs_b) {
   return int(ceilf(2.0f * fmaxf(hs_a, hs_b) + 1.0e-3f)) + 1;
 }
 
-// v weights of the column, broadcast from lanes.
+// Lane L receives v from lane L + 1 (DPP wave shift; the last lane's value
+// is unused).
+__device__ __forceinline__ float from_next_lane(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130 /* wave_shl:1 */,
+                                         0xF, 0xF, false));
+}
+
+// v weights of the column: lane m evaluates the CDF at the m-th v pixel
+// bound, one shifted difference gives the weight of pixel m in lane m (zero
+// from the window's end on), and readlane broadcasts the W weights.
 template <int W, typename T>
 __device__ __forceinline__ void vweights(const Col<T> &c, int lane, T (&cv)[W]) {
   const T f = cdf3c<T>(T(double(c.vl + lane) - c.vc) * c.sv);
-  const int n = c.vr - c.vl;
-  T prev = bcast(f, 0);
+  // the shift runs on all lanes (a lane reading a disabled lane gets 0)
+  const T fn = from_next_lane(f);
+  const T d = lane < c.vr - c.vl ? fn - f : T(0);
 #pragma unroll
-  for (int m = 0; m < W; ++m) {
-    const T nx = bcast(f, m + 1);
-    cv[m] = m < n ? nx - prev : T(0);
-    prev = nx;
-  }
+  for (int m = 0; m < W; ++m) cv[m] = bcast(d, m);
 }
 
 // Row loops with a wave-uniform row count N (1..W), dispatched once per
