@@ -19,6 +19,7 @@
 //    voxel's axial weights against Q.
 // Voxels whose window leaves the column hull take a direct global path.
 #include <climits>
+#include <type_traits>
 
 #include "trtt_core.h"
 
@@ -640,8 +641,15 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     const bool any = ur > ul && vr > vl;
     const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && RW * RH <= wcap;
-    if (in_lds)
-      for (int idx = lane; idx < RW * RH; idx += 64) win[idx] = T(0);
+    // window zeroing and flush in 16-byte vectors: the capacity wcap is a
+    // multiple of 64 elements, so the rounded-up count stays inside it
+    using V16 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int kE = 16 / int(sizeof(T));
+    const int nvec = (RW * RH + kE - 1) / kE;
+    if (in_lds) {
+      V16 *w16 = reinterpret_cast<V16 *>(win);
+      for (int q = lane; q < nvec; q += 64) w16[q] = V16{};
+    }
     __builtin_amdgcn_wave_barrier();
 
     if (any) {
@@ -711,8 +719,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       }
     }
     if (in_lds) {
-      T *dst = scratch + slot * wcap;
-      for (int idx = lane; idx < RW * RH; idx += 64) dst[idx] = win[idx];
+      V16 *dst = reinterpret_cast<V16 *>(scratch + slot * wcap);
+      const V16 *w16 = reinterpret_cast<const V16 *>(win);
+      for (int q = lane; q < nvec; q += 64) dst[q] = w16[q];
     }
     if (lane == 0) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
     __builtin_amdgcn_wave_barrier();
@@ -885,9 +894,20 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
       const bool in_lds = any && RW * RH <= lds_cap;
       if (in_lds) {
+        // flat over the window with the source offset advanced
+        // incrementally (one division per view, not per element)
+        const int sr = kThreads / RW, sc = kThreads - sr * RW;
+        int r = int(threadIdx.x) / RW, cI = int(threadIdx.x) - r * RW;
+        long off = long(vl + r) * P.nu + (ul + cI);
+        const long step = long(sr) * P.nu + sc, wrap = long(P.nu) - RW;
         for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) {
-          const int r = idx / RW, cI = idx - r * RW;
-          region[idx] = in[long(vl + r) * P.nu + (ul + cI)];
+          region[idx] = in[off];
+          off += step;
+          cI += sc;
+          if (cI >= RW) {
+            cI -= RW;
+            off += wrap;
+          }
         }
       }
       __syncthreads();
