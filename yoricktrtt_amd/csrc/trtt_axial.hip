@@ -140,14 +140,17 @@ struct Vox {
   bool clipped;  // left bound clamped at the detector edge
 };
 
+// Pixel window [lo, hi) of the support [c - hs, c + hs] on a detector of n
+// pixels.  A support wholly past either edge gives lo >= hi (nothing to
+// add); float-to-int conversion saturates, and the right end is capped so
+// that hx + 1 cannot overflow.
 __device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int &hi, int &hx,
                                          bool &clipped) {
   const float l = c - hs, r = c + hs;
-  const float nm1 = float(n - 1);
-  lo = l < 0.0f ? 0 : (l <= nm1 ? int(floorf(l)) : n - 1);
-  hx = int(floorf(fminf(fmaxf(r, -1.0f), 1.0e7f))) + 1;
-  hi = r < 0.0f ? 0 : min(hx, n);
-  clipped = !(l >= 0.0f && l <= nm1);
+  lo = max(int(floorf(l)), 0);
+  hx = int(floorf(fminf(r, 1.0e7f))) + 1;
+  hi = min(hx, n);
+  clipped = l < 0.0f;
 }
 
 template <typename T>
@@ -534,14 +537,18 @@ __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> 
   c.hu0 = bcast(src.c.hu0, cc);
   c.inv_a = bcast(src.c.inv_a, cc);
   c.sv = bcast(src.c.sv, cc);
-  c.vl = bcast_i(src.c.vl, cc);
-  c.vr = bcast_i(src.c.vr, cc);
+  // index pairs packed in 16 bits each (the host limits the sizes)
+  const int v2 = bcast_i(src.c.vl | (src.c.vr << 16), cc);
+  c.vl = v2 & 0xffff;
+  c.vr = int(unsigned(v2) >> 16);
   c.ok = true;
-  lo = bcast_i(src.lo, cc);
-  hi = bcast_i(src.hi, cc);
+  const int h2 = bcast_i(src.lo | (src.hi << 16), cc);
+  lo = h2 & 0xffff;
+  hi = int(unsigned(h2) >> 16);
   np = bcast_i(src.np, cc);
-  i = bcast_i(src.i, cc);
-  j = bcast_i(src.j, cc);
+  const int ij = bcast_i(src.i | (src.j << 16), cc);
+  i = ij & 0xffff;
+  j = int(unsigned(ij) >> 16);
 }
 
 __device__ __forceinline__ int wred_min(int x) {
@@ -661,7 +668,8 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       unsigned long long pend = __ballot(cl.live);
       T xa = T(0), xb = T(0);
       auto load_col = [&](int col) {
-        const long idx = (long(bcast_i(cl.j, col)) * P.nx + bcast_i(cl.i, col)) * P.nz + kc;
+        const int ij = bcast_i(cl.i | (cl.j << 16), col);
+        const long idx = (long(unsigned(ij) >> 16) * P.nx + (ij & 0xffff)) * P.nz + kc;
         xa = vol0[idx];
         xb = two ? vol1[idx] : T(0);
       };
@@ -822,15 +830,21 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
     }
     __syncthreads();
     const int n = nhit;
+    // unconditional loads (threads outside a window read its first entry
+    // and add 0) so that the loads of several hits are in flight together
+#pragma unroll 4
     for (int q0 = 0; q0 < n; ++q0) {
       const int tt = hit_t[q0];
       const int4 r = hit_r[q0];
       const T *src = scratch + (long(vloc) * ntiles + tt) * cap;
-      const bool uin = u >= r.x && u < r.x + r.z;
+      const int du = u - r.x;
+      const bool uin = du >= 0 && du < r.z;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int v = v0 + ty + 4 * q;
-        if (uin && v >= r.y && v < r.y + r.w) acc[q] += src[(v - r.y) * r.z + (u - r.x)];
+        const int dv = v0 + ty + 4 * q - r.y;
+        const bool in = uin && dv >= 0 && dv < r.w;
+        const T val = src[in ? dv * r.z + du : 0];
+        acc[q] += in ? val : T(0);
       }
     }
     __syncthreads();
