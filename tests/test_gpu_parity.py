@@ -394,3 +394,49 @@ def test_axial_wide_footprints_and_truncation(lib, oracle):
         oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v], geo.xsd[v], True, 1)
     assert rel_err(bwd, rb) < 1e-5
     P.close()
+
+
+@pytest.mark.parametrize("nt", [1, 3])
+def test_axial_backward_tiles_vs_oracle(lib, oracle, nt):
+    """Every backward launch shape, including the tiles of 128 slices (two
+    voxels per lane), against the FP64 oracle: a slice count that leaves a
+    partial last tile for both k extents, and (nt = 3) views interpolated
+    between two phases next to one-hot ones."""
+    from yoricktrtt_amd import autotune as AT
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    nx, ny, nz, nu, nv, V = 20, 18, 150, 250, 40, 8
+    _say("backward tiles", nt)
+    rng = np.random.default_rng(40 + nt)
+    th = rng.uniform(0, 2 * math.pi, size=V)
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, th, 300.0, 450.0)
+    kw = {}
+    t0 = np.zeros(V, np.int32)
+    t1 = np.zeros(V, np.int32)
+    w0 = np.ones(V)
+    w1 = np.zeros(V)
+    if nt > 1:
+        t = rng.uniform(0, nt - 1, size=V)
+        t[:3] = np.floor(t[:3])
+        t0 = np.floor(t).astype(np.int32)
+        t1 = np.minimum(t0 + 1, nt - 1).astype(np.int32)
+        w1 = t - t0
+        w1[t0 == t1] = 0.0
+        w0 = 1.0 - w1
+        kw = dict(nt=nt, t0=t0, t1=t1, w0=w0, w1=w1)
+    P = Projector(geo, model="cubic", precision="f32", **kw)
+    assert P.path > 0
+    y = rng.uniform(size=(V, nv, nu))
+    rb = np.zeros((nt, nz, ny, nx))
+    for v in range(V):
+        bv = oracle.sb3d(np.zeros((nz, ny, nx)), y[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0,
+                         geo.xos[v], geo.xsd[v], True, 1)
+        rb[t0[v]] += w0[v] * bv
+        rb[t1[v]] += w1[v] * bv
+    for tile in sorted(AT.BWD_TILES):
+        AT.set_tuning(P, bwd_tile=tile, bwd_window_bytes=32768)
+        bwd = P.backward(_dev(y, torch.float32)).double().cpu().numpy().reshape(rb.shape)
+        assert rel_err(bwd, rb) < 1e-5, tile
+    P.close()
+

@@ -244,7 +244,7 @@ def test_autotune_candidates_and_cache(tmp_path):
     cache = str(tmp_path / "tuning.json")
     r = AT.autotune(P, cache=cache, reps=1, windows=(8192, 16384), vpbs=(0, 2), chunks=(0, 3))
     assert not r["cached"] and AT.get_tuning(P) == r["tuning"]
-    assert len(r["rows"]) == 3 * 2 * 2 * 2 + 9
+    assert len(r["rows"]) == 3 * 2 * 2 * 2 + sum(len(w) for w in AT.BWD_WINDOWS.values())
     r2 = AT.autotune(P, cache=cache)
     assert r2["cached"] and r2["tuning"] == r["tuning"]
     with pytest.raises(RuntimeError):

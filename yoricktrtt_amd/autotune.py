@@ -23,13 +23,16 @@ import time
 from . import _lib
 
 FWD_TILES = {0: "8x4", 1: "4x4", 2: "8x8"}
-BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4"}
+# backward tiles 3 and 4 span 128 slices (two voxels per lane); plans whose
+# column profiles over 128 slices exceed 256 pixels reject them
+BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4", 3: "8x4x128", 4: "4x4x128"}
 # LDS window sizes tried per forward tile (a window too small for a tile's
 # detector footprint sends its voxels down the slow global-atomic path)
 WINDOWS = {0: (12288, 16384, 24576), 1: (8192, 12288, 16384), 2: (24576, 32768)}
 VPBS = (0, 1, 2, 4)
 CHUNKS = (0, 2, 4)  # views per forward scratch round (0: as many as fit the budget)
-BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384)}
+BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384),
+               3: (24576, 32768), 4: (16384, 24576)}
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "yoricktrtt_amd", "tuning.json")
 
 
@@ -157,7 +160,11 @@ def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vp
             best_f = (c, t)
     set_tuning(P, **best_f[0])
     for c in bwd_c:
-        set_tuning(P, **c)
+        try:
+            set_tuning(P, **c)
+        except RuntimeError:  # a tile the plan's geometry does not allow
+            rows.append(dict(kind="bwd", **c, ms=None, rejected=True))
+            continue
         t1 = once(lambda: P.backward(y, vol.zero_()))
         err = float((vol - ref_b).abs().max()) / scale_b
         if err > 1e-5:

@@ -860,12 +860,16 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
 // ---------------------------------------------------------------------------
 // Backward
 // ---------------------------------------------------------------------------
-template <bool CONE, int W, int CX, int CY, bool TWO, typename T>
+// VPL voxels per lane: the tile spans 64 VPL slices and lane L holds the
+// voxels k = kf + 64 q + L, q < VPL, so that the per-column set-up (column
+// broadcast, v weights, Q row) serves 64 VPL voxels.
+template <bool CONE, int W, int CX, int CY, bool TWO, int VPL, typename T>
 __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T *__restrict__ sino,
                                                 T *__restrict__ vol,
                                                 const int *__restrict__ seg_begin, int atomic_out,
                                                 int lds_cap, int pmax) {
   constexpr int NCOL = CX * CY / kWaves;
+  constexpr int NA = NCOL * VPL;  // accumulators per lane
   static_assert(NCOL <= 64, "one column per lane");
   T *region = reinterpret_cast<T *>(g_smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -873,20 +877,18 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
   __shared__ int red[4 * kWaves];
   TileMap tm;
   tm.init(P, CX, CY);
-  const int k = tm.bz * kKT + lane;
-  const bool kin = k < P.nz;
-  const int kf = tm.bz * kKT, kl = min(kf + kKT - 1, P.nz - 1);
+  const int kf = tm.bz * (kKT * VPL), kl = min(kf + kKT * VPL - 1, P.nz - 1);
   const long npix = long(P.nu) * P.nv;
   const int s_begin = seg_begin[blockIdx.y], s_end = seg_begin[blockIdx.y + 1];
 
   for (int s = s_begin; s < s_end; ++s) {
     const Segment seg = P.segs[s];
     // TWO: some segment interpolates between two phases (second accumulator)
-    T acc0[NCOL], acc1[TWO ? NCOL : 1];
+    T acc0[NA], acc1[TWO ? NA : 1];
 #pragma unroll
-    for (int cc = 0; cc < NCOL; ++cc) acc0[cc] = T(0);
+    for (int a = 0; a < NA; ++a) acc0[a] = T(0);
 #pragma unroll
-    for (int cc = 0; cc < (TWO ? NCOL : 1); ++cc) acc1[cc] = T(0);
+    for (int a = 0; a < (TWO ? NA : 1); ++a) acc1[a] = T(0);
 
     for (int n = 0; n < seg.count; ++n) {
       const int view = P.order[seg.start + n];
@@ -939,70 +941,95 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
           const bool use_q = in_lds && span <= pmax && npv <= W;
           T cv[W];
           vweights<W>(c, lane, cv);
-          T contrib = T(0);
+          T contrib[VPL];
+#pragma unroll
+          for (int q = 0; q < VPL; ++q) contrib[q] = T(0);
           if (use_q) {
             const T *src0 = region + (c.vl - vl) * RW + (lo - ul);
-            if (span <= 128) {
-              // both halves in one batch of reads; lanes past the span read
-              // other entries of the LDS region (index < lds_cap + 128) and
+            if (span <= 128 * VPL) {
+              // 128 pixels per batch of reads; lanes past the span read other
+              // entries of the LDS region (index < lds_cap + 128 VPL) and
               // store zeros into the slack
-              T q0, q1;
-              q_n2<W>(npv, src0 + lane, RW, cv, q0, q1);
-              if (lane < span + W) qrow[lane] = lane < span ? q0 : T(0);
-              if (lane + 64 < span + W) qrow[lane + 64] = lane + 64 < span ? q1 : T(0);
+#pragma unroll
+              for (int h = 0; h < VPL; ++h) {
+                const int b = 128 * h;
+                if (h == 0 || span > b) {
+                  T q0, q1;
+                  q_n2<W>(npv, src0 + b + lane, RW, cv, q0, q1);
+                  if (b + lane < span + W) qrow[b + lane] = b + lane < span ? q0 : T(0);
+                  if (b + lane + 64 < span + W)
+                    qrow[b + lane + 64] = b + lane + 64 < span ? q1 : T(0);
+                }
+              }
             } else {
               for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, RW, cv);
             }
             if (lane < W) qrow[span + lane] = T(0);
             __builtin_amdgcn_wave_barrier();
-            {
+#pragma unroll
+            for (int q = 0; q < VPL; ++q) {
+              const int k = kf + kKT * q + lane;
+              const bool kin = k < P.nz;
               Vox<T> v;
               voxel(P, a12, c, kin ? k : P.nz - 1, v);
               if (kin && v.ul < v.ur) {
                 if (v.ul >= lo && v.ur <= hi && fits(v, np, P.nu)) {
                   // all np pixels: zero weights past the footprint, zero Q
                   // past the column span (right detector edge)
-                  contrib = gather_n<W>(np, v, qrow + (v.ul - lo)) * v.rho;
+                  contrib[q] = gather_n<W>(np, v, qrow + (v.ul - lo)) * v.rho;
                 } else {
-                  slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
+                  slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
                 }
               }
             }
             __builtin_amdgcn_wave_barrier();
-          } else if (kin) {
-            Vox<T> v;
-            voxel(P, a12, c, k, v);
-            if (v.ul < v.ur) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib);
-          }
-          if (!TWO || seg.t0 == seg.t1) {
-            acc0[cc] += wsum * contrib;
           } else {
-            acc0[cc] += w0 * contrib;
-            acc1[TWO ? cc : 0] += w1 * contrib;
+#pragma unroll
+            for (int q = 0; q < VPL; ++q) {
+              const int k = kf + kKT * q + lane;
+              if (k < P.nz) {
+                Vox<T> v;
+                voxel(P, a12, c, k, v);
+                if (v.ul < v.ur) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+              }
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < VPL; ++q) {
+            if (!TWO || seg.t0 == seg.t1) {
+              acc0[cc * VPL + q] += wsum * contrib[q];
+            } else {
+              acc0[cc * VPL + q] += w0 * contrib[q];
+              acc1[TWO ? cc * VPL + q : 0] += w1 * contrib[q];
+            }
           }
         }
       }
       __syncthreads();
     }
-    if (kin) {
+#pragma unroll
+    for (int q = 0; q < VPL; ++q) {
+      const int k = kf + kKT * q + lane;
+      if (k >= P.nz) continue;
 #pragma unroll
       for (int cc = 0; cc < NCOL; ++cc) {
         const int col = wave * NCOL + cc;
         const int i = tm.bx * CX + col % CX, j = tm.by * CY + col / CX;
         if (i >= P.nx || j >= P.ny) continue;
+        const int a = cc * VPL + q;
         const long idx = (long(k) * P.ny + j) * P.nx + i;
         T *d0 = vol + long(seg.t0) * P.vox_per_phase + idx;
         if (atomic_out) {
-          if (acc0[cc] != T(0)) unsafeAtomicAdd(d0, acc0[cc]);
+          if (acc0[a] != T(0)) unsafeAtomicAdd(d0, acc0[a]);
         } else {
-          *d0 += acc0[cc];
+          *d0 += acc0[a];
         }
         if (TWO && seg.t0 != seg.t1) {
           T *d1 = vol + long(seg.t1) * P.vox_per_phase + idx;
           if (atomic_out) {
-            if (acc1[TWO ? cc : 0] != T(0)) unsafeAtomicAdd(d1, acc1[TWO ? cc : 0]);
+            if (acc1[TWO ? a : 0] != T(0)) unsafeAtomicAdd(d1, acc1[TWO ? a : 0]);
           } else {
-            *d1 += acc1[TWO ? cc : 0];
+            *d1 += acc1[TWO ? a : 0];
           }
         }
       }
@@ -1012,19 +1039,21 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 
 }  // namespace axial
 
-// Tile shapes (columns along x, y; each tile spans 64 k) selectable per
-// plan: forward one wave per workgroup, backward four waves.
+// Tile shapes (columns along x, y; forward tiles span 64 k, backward tiles
+// 64 k times bwd_tile_vpl) selectable per plan: forward one wave per
+// workgroup, backward four waves.
 constexpr int kFwdTiles[kNumFwdTiles][2] = {{8, 4}, {4, 4}, {8, 8}};
-constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}};
+constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}, {8, 4}, {4, 4}};
 constexpr int kAxialPmax = 256;  // longest column profile held in LDS
 
-int axial_tiles(const DevPlan &P, int cx, int cy) {
-  return ((P.nx + cx - 1) / cx) * ((P.ny + cy - 1) / cy) * ((P.nz + axial::kKT - 1) / axial::kKT);
+int axial_tiles(const DevPlan &P, int cx, int cy, int kt) {
+  return ((P.nx + cx - 1) / cx) * ((P.ny + cy - 1) / cy) * ((P.nz + kt - 1) / kt);
 }
 
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu) {
-  const int *t = job == 0 ? kFwdTiles[tu.fwd_tile] : kBwdTiles[tu.bwd_tile];
-  return axial_tiles(P, t[0], t[1]);
+  if (job == 0) return axial_tiles(P, kFwdTiles[tu.fwd_tile][0], kFwdTiles[tu.fwd_tile][1], axial::kKT);
+  return axial_tiles(P, kBwdTiles[tu.bwd_tile][0], kBwdTiles[tu.bwd_tile][1],
+                     axial::kKT * bwd_tile_vpl(tu.bwd_tile));
 }
 
 // Scratch of the forward reduction for `chunk` views: one detector window
@@ -1052,22 +1081,22 @@ void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, cons
                      vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
 }
 
-template <bool CONE, int W, int CX, int CY, typename T>
+template <bool CONE, int W, int CX, int CY, int VPL, typename T>
 void launch_bwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *in, T *out,
                 const int *seg_begin, int atomic_out, int lds_cap, int pmax, bool two) {
   if (two)
-    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, true, T>), grid, dim3(axial::kThreads),
+    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, true, VPL, T>), grid, dim3(axial::kThreads),
                        lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax);
   else
-    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, false, T>), grid, dim3(axial::kThreads),
+    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, false, VPL, T>), grid, dim3(axial::kThreads),
                        lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax);
 }
 
 template <bool CONE, int W, typename T>
 hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
                           const int *seg_begin, int nrows, int views_per_block, int atomic_out,
-                          int pmax, void *scratch, void *hdr, int chunk, void *volk_buf,
-                          int plain, const AxialTuning &tu, hipEvent_t *ev,
+                          int pmax, int pmax2, void *scratch, void *hdr, int chunk,
+                          void *volk_buf, int plain, const AxialTuning &tu, hipEvent_t *ev,
                           const AxialOverlap *ov, int nbuf, int two, hipStream_t st) {
   // ev (optional): pairs of events around each main kernel launch
   int npair = 0;
@@ -1141,27 +1170,38 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
       hipStreamWaitEvent(st, ov->red_done[1], 0);
     }
   } else {
+    const int vpl = bwd_tile_vpl(tu.bwd_tile);
+    if (vpl == 2 && pmax2 <= 0) return hipErrorInvalidValue;
+    const int pm = vpl == 2 ? pmax2 : pmax;
     const int lds_cap = tu.bwd_window_bytes / int(sizeof(T));
-    // window, one Q row with W zero entries per wave; at least 128 entries
-    // past the window for the batched Q of spans up to 128 pixels
+    // window, one Q row with W zero entries per wave; at least 128 VPL
+    // entries past the window for the batched Q rows
     const int lds_bytes =
-        tu.bwd_window_bytes + max(axial::kWaves * (pmax + W), 128) * int(sizeof(T));
+        tu.bwd_window_bytes + max(axial::kWaves * (pm + W), 128 * vpl) * int(sizeof(T));
     dim3 grid(unsigned(axial_tiles(P, 1, tu)), nrows);
     const auto *i = static_cast<const T *>(in);
     auto *o = static_cast<T *>(out);
     if (ev) hipEventRecord(ev[0], st);
     switch (tu.bwd_tile) {
       case 1:
-        launch_bwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
-                                     pmax, two != 0);
+        launch_bwd<CONE, W, 8, 4, 1, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
+                                        lds_cap, pm, two != 0);
         break;
       case 2:
-        launch_bwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
-                                     pmax, two != 0);
+        launch_bwd<CONE, W, 4, 4, 1, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
+                                        lds_cap, pm, two != 0);
+        break;
+      case 3:
+        launch_bwd<CONE, W, 8, 4, 2, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
+                                        lds_cap, pm, two != 0);
+        break;
+      case 4:
+        launch_bwd<CONE, W, 4, 4, 2, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
+                                        lds_cap, pm, two != 0);
         break;
       default:
-        launch_bwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out, lds_cap,
-                                     pmax, two != 0);
+        launch_bwd<CONE, W, 8, 8, 1, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
+                                        lds_cap, pm, two != 0);
     }
     if (ev) hipEventRecord(ev[1], st);
   }
@@ -1170,7 +1210,7 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
 
 hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
                         void *out, const int *seg_begin, int nrows, int views_per_block,
-                        int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
+                        int atomic_out, int pmax, int pmax2, void *scratch, void *hdr, int chunk,
                         void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
                         const AxialOverlap *ov, int nbuf, int two, hipStream_t st) {
   if (tu.fwd_tile < 0 || tu.fwd_tile >= kNumFwdTiles || tu.bwd_tile < 0 ||
@@ -1179,8 +1219,8 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
 #define AX_CASE(C, WW, TT)                                                                   \
   if (cone == C && w == WW && f64 == (sizeof(TT) == 8))                                     \
     return launch_axial_t<C, WW, TT>(P, job, in, out, seg_begin, nrows, views_per_block,   \
-                                     atomic_out, pmax, scratch, hdr, chunk, volk, plain, tu, ev, \
-                                     ov, nbuf, two, st);
+                                     atomic_out, pmax, pmax2, scratch, hdr, chunk, volk, plain, \
+                                     tu, ev, ov, nbuf, two, st);
   AX_CASE(true, 9, float)
   AX_CASE(true, 13, float)
   AX_CASE(false, 9, float)

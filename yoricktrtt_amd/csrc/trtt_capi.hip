@@ -20,8 +20,8 @@ hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long
                            hipStream_t st);
 hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
                         void *out, const int *seg_begin, int nrows, int views_per_block,
-                        int atomic_out, int pmax, void *scratch, void *hdr, int chunk,
-                        void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
+                        int atomic_out, int pmax, int pmax2, void *scratch, void *hdr,
+                        int chunk, void *volk, int plain, const AxialTuning &tu, hipEvent_t *ev,
                         const AxialOverlap *ov, int nbuf, int two, hipStream_t st);
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu);
 size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
@@ -168,6 +168,7 @@ void view_consts_lf(ViewConst &g, const lf2d_parameters_t *p, double beta, doubl
 void axial_analysis(trtt_hip_plan *p) {
   p->axial_w = 0;
   p->axial_pmax = 0;
+  p->axial_pmax2 = 0;
   if (p->table || p->precision != TRTT_HIP_F32) return;
   if (p->kind != trtt::CB3D && p->kind != trtt::PB3D) return;
   // the kernels broadcast detector and column indices packed in 16 bits
@@ -220,6 +221,9 @@ void axial_analysis(trtt_hip_plan *p) {
   if (w == 0 || !(pmax <= 256.0)) return;
   p->axial_w = w;
   p->axial_pmax = int(pmax);
+  // backward tiles of 128 k: the profile over 128 slices must fit too
+  const double pmax2 = std::ceil(127.0 * duc_max + 2.0 * hs0 * hu_max) + 8.0;
+  p->axial_pmax2 = pmax2 <= 256.0 ? int(pmax2) : 0;
   // consecutive voxels of a column start their windows on distinct pixels
   p->axial_plain = duc_min - 2.0 * dhs_rate > 1.01;
   // TRTT_HIP_PROFILE_ATOMIC=1 accumulates the axial profiles with LDS
@@ -591,7 +595,7 @@ hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hip
     }
     return trtt::launch_axial(p->kind == trtt::CB3D, p->axial_w, f64, p->dev, job, in, out,
                               p->d_seg_begin, nrows, int(vpb), nrows > 1 ? 1 : 0, p->axial_pmax,
-                              p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
+                              p->axial_pmax2, p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
                               p->axial_plain ? 1 : 0, p->tune, ev, &p->overlap, p->axial_nbuf,
                               p->two_phase_segs ? 1 : 0, st);
   }
@@ -758,6 +762,8 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
     return fail("backward window must be 4096..131072 bytes, a multiple of 256");
   if (t->fwd_chunk_views < 0) return fail("views per forward chunk must be >= 0");
   if (p->axial_w <= 0) return TRTT_SUCCESS;  // general kernels: nothing to tune
+  if (trtt::bwd_tile_vpl(t->bwd_tile) == 2 && p->axial_pmax2 == 0)
+    return fail("backward tiles of 128 slices need column profiles of at most 256 pixels");
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return TRTT_FAILURE;
   const trtt::AxialTuning old = p->tune;

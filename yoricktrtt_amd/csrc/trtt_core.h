@@ -63,7 +63,9 @@ struct DevPlan {
 
 // Launch shape of the axial kernels of a plan (trtt_hip_plan_set_tuning).
 constexpr int kNumFwdTiles = 3;  // 8x4, 4x4, 8x8 columns x 64 k
-constexpr int kNumBwdTiles = 3;  // 8x8, 8x4, 4x4 columns x 64 k
+constexpr int kNumBwdTiles = 5;  // 8x8, 8x4, 4x4 columns x 64 k; 8x4, 4x4 x 128 k
+// voxels per lane of a backward tile (its k extent is 64 times this)
+constexpr int bwd_tile_vpl(int t) { return t >= 3 ? 2 : 1; }
 struct AxialTuning {
   int fwd_tile = 0;
   int bwd_tile = 0;
@@ -110,6 +112,7 @@ struct trtt_hip_plan {
   // footprint width bound W (0: not eligible) and column profile length
   int axial_w = 0;
   int axial_pmax = 0;
+  int axial_pmax2 = 0;       // column profile length over 128 k (0: over 256)
   bool axial_plain = false;   // profile accumulation without LDS atomics
   int axial_chunk = 1;      // views per forward scratch round
   void *d_scratch = nullptr;  // forward windows, one per (view, tile)
