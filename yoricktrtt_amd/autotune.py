@@ -23,16 +23,16 @@ import time
 from . import _lib
 
 FWD_TILES = {0: "8x4", 1: "4x4", 2: "8x8"}
-# backward tiles 3 and 4 span 128 slices (two voxels per lane); plans whose
+# backward tiles 3 to 5 span 128 slices (two voxels per lane); plans whose
 # column profiles over 128 slices exceed 256 pixels reject them
-BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4", 3: "8x4x128", 4: "4x4x128"}
+BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4", 3: "8x4x128", 4: "4x4x128", 5: "8x8x128"}
 # LDS window sizes tried per forward tile (a window too small for a tile's
 # detector footprint sends its voxels down the slow global-atomic path)
 WINDOWS = {0: (12288, 16384, 24576), 1: (8192, 12288, 16384), 2: (24576, 32768)}
 VPBS = (0, 1, 2, 4)
 CHUNKS = (0, 2, 4)  # views per forward scratch round (0: as many as fit the budget)
 BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384),
-               3: (24576, 32768), 4: (16384, 24576)}
+               3: (24576, 32768), 4: (16384, 24576), 5: (32768, 49152)}
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "yoricktrtt_amd", "tuning.json")
 
 

@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 // 64 k times bwd_tile_vpl) selectable per plan: forward one wave per
 // workgroup, backward four waves.
 constexpr int kFwdTiles[kNumFwdTiles][2] = {{8, 4}, {4, 4}, {8, 8}};
-constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}, {8, 4}, {4, 4}};
+constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}, {8, 4}, {4, 4}, {8, 8}};
 constexpr int kAxialPmax = 256;  // longest column profile held in LDS
 
 int axial_tiles(const DevPlan &P, int cx, int cy, int kt) {
@@ -1197,6 +1197,10 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
         break;
       case 4:
         launch_bwd<CONE, W, 4, 4, 2, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
+                                        lds_cap, pm, two != 0);
+        break;
+      case 5:
+        launch_bwd<CONE, W, 8, 8, 2, T>(P, grid, lds_bytes, st, i, o, seg_begin, atomic_out,
                                         lds_cap, pm, two != 0);
         break;
       default:

@@ -63,7 +63,7 @@ struct DevPlan {
 
 // Launch shape of the axial kernels of a plan (trtt_hip_plan_set_tuning).
 constexpr int kNumFwdTiles = 3;  // 8x4, 4x4, 8x8 columns x 64 k
-constexpr int kNumBwdTiles = 5;  // 8x8, 8x4, 4x4 columns x 64 k; 8x4, 4x4 x 128 k
+constexpr int kNumBwdTiles = 6;  // 8x8, 8x4, 4x4 columns x 64 k; 8x4, 4x4, 8x8 x 128 k
 // voxels per lane of a backward tile (its k extent is 64 times this)
 constexpr int bwd_tile_vpl(int t) { return t >= 3 ? 2 : 1; }
 struct AxialTuning {
