@@ -641,12 +641,14 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
     if (cl.bad) atomicOr(P.err, 1);
     // wave-uniform (readfirstlane: kept in scalar registers)
-    const int ul = bcast_i(wred_min(cl.live ? cl.lo : INT_MAX), 0);
+    // the window starts on a multiple of 4 pixels and its rows are a
+    // multiple of 4 long (16-byte loads in the reduction)
+    const int ul = bcast_i(wred_min(cl.live ? cl.lo : INT_MAX), 0) & ~3;
     const int ur = bcast_i(wred_max(cl.live ? cl.hi : INT_MIN), 0);
     const int vl = bcast_i(wred_min(cl.live ? cl.c.vl : INT_MAX), 0);
     const int vr = bcast_i(wred_max(cl.live ? cl.c.vr : INT_MIN), 0);
     const bool any = ur > ul && vr > vl;
-    const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
+    const int RW = any ? (ur - ul + 3) & ~3 : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && RW * RH <= wcap;
     // window zeroing and flush in 16-byte vectors: the capacity wcap is a
     // multiple of 64 elements, so the rounded-up count stays inside it
@@ -804,10 +806,15 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
   const int npu = (P.nu + kRedU - 1) / kRedU, npv = (P.nv + kRedV - 1) / kRedV;
   const int pu = blockIdx.x % npu, pv = blockIdx.x / npu;
   const int vloc = blockIdx.y;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int u = pu * kRedU + tx, v0 = pv * kRedV;
+  // 16 x 16 threads, each on 4 consecutive pixels of one row: windows start
+  // on a multiple of 4 pixels and their rows are multiples of 4 long, so the
+  // four pixels are inside or outside a window together and one 16-byte
+  // load reads them
+  static_assert(sizeof(T) == 4, "four pixels per 16-byte load");
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int u = pu * kRedU + 4 * tx, v0 = pv * kRedV, v = v0 + ty;
   const int u_lo = pu * kRedU, u_hi = u_lo + kRedU;
-  T acc[4] = {T(0), T(0), T(0), T(0)};
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int4 *h = hdr + long(vloc) * ntiles;
   const long b = long(vloc) * npu * npv + blockIdx.x;
   const int nlist = count[b];
@@ -830,30 +837,31 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
     }
     __syncthreads();
     const int n = nhit;
-    // unconditional loads (threads outside a window read its first entry
+    // unconditional loads (threads outside a window read its first entries
     // and add 0) so that the loads of several hits are in flight together
 #pragma unroll 4
     for (int q0 = 0; q0 < n; ++q0) {
       const int tt = hit_t[q0];
       const int4 r = hit_r[q0];
       const T *src = scratch + (long(vloc) * ntiles + tt) * cap;
-      const int du = u - r.x;
-      const bool uin = du >= 0 && du < r.z;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int dv = v0 + ty + 4 * q - r.y;
-        const bool in = uin && dv >= 0 && dv < r.w;
-        const T val = src[in ? dv * r.z + du : 0];
-        acc[q] += in ? val : T(0);
+      const int du = u - r.x, dv = v - r.y;
+      const bool in = du >= 0 && du < r.z && dv >= 0 && dv < r.w;
+      const float4 val = *reinterpret_cast<const float4 *>(src + (in ? dv * r.z + du : 0));
+      if (in) {
+        acc.x += val.x;
+        acc.y += val.y;
+        acc.z += val.z;
+        acc.w += val.w;
       }
     }
     __syncthreads();
   }
-  T *out = sino + long(view_base + vloc) * P.nu * P.nv;
+  if (v < P.nv) {
+    T *row = sino + long(view_base + vloc) * P.nu * P.nv + long(v) * P.nu;
+    const float a[4] = {acc.x, acc.y, acc.z, acc.w};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int v = v0 + ty + 4 * q;
-    if (u < P.nu && v < P.nv) out[long(v) * P.nu + u] += acc[q];
+    for (int e = 0; e < 4; ++e)
+      if (u + e < P.nu) row[u + e] += a[e];
   }
 }
 
