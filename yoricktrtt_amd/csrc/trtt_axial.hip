@@ -82,6 +82,7 @@ __device__ __forceinline__ void clampw(double c, double hs, int n, int &lo, int 
 template <typename T>
 struct Col {
   double vc;       // transaxial centre
+  T vcf;           // vc - vl (single precision is exact enough: |vc - vl| < a few pixels)
   double uc0, duc; // uc(k) = uc0 + duc k
   T rho;           // rho0 / du
   T hu0;           // hu / du
@@ -114,6 +115,7 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.hu0 = T(P.hu_init * magn);
     c.inv_a = T(1.0 / a);
     clampw(c.vc, P.hs0 * hv, P.nv, c.vl, c.vr);
+    c.vcf = T(c.vc - double(c.vl));
   } else {
     c.ok = true;
     c.vc = (g.nvc + g.nvy * dj) + g.nvx * di;
@@ -124,6 +126,7 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.hu0 = T(g.pb_hu);
     c.inv_a = T(0);
     clampw(c.vc, g.pb_hsv, P.nv, c.vl, c.vr);
+    c.vcf = T(c.vc - double(c.vl));
   }
 }
 
@@ -216,7 +219,7 @@ __device__ __forceinline__ float from_next_lane(float v) {
 // from the window's end on), and readlane broadcasts the W weights.
 template <int W, typename T>
 __device__ __forceinline__ void vweights(const Col<T> &c, int lane, T (&cv)[W]) {
-  const T f = cdf3c<T>(T(double(c.vl + lane) - c.vc) * c.sv);
+  const T f = cdf3c<T>((T(lane) - c.vcf) * c.sv);
   // the shift runs on all lanes (a lane reading a disabled lane gets 0)
   const T fn = from_next_lane(f);
   const T d = lane < c.vr - c.vl ? fn - f : T(0);
@@ -418,7 +421,7 @@ __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T
   const int npu = v.ur - v.ul, npv = c.vr - c.vl;
   if (npu <= 0 || npv <= 0) return;
   const T xu0 = T(double(v.ul) - v.uc) * v.su;
-  const T xv0 = T(double(c.vl) - c.vc) * c.sv;
+  const T xv0 = -c.vcf * c.sv;
   T fv0 = cdf3c<T>(xv0);
   T sum = T(0);
   for (int m = 0; m < npv; ++m) {
@@ -501,6 +504,7 @@ struct ColLane {
   int lo, hi;   // u hull of the column over [kf, kl]
   int np;       // pixel bound of its voxel windows
   int i, j;     // voxel column
+  int base;     // (j nx + i) nz: the column's offset in the k-fastest volume copy
   bool live;    // inside the volume, wk != 0, non-empty windows
   bool bad;     // wk == 0 (voxel in the source plane)
 };
@@ -511,6 +515,7 @@ __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &
   const int col = wave * NCOL + lane;
   cl.i = tm.bx * CX + col % CX;
   cl.j = tm.by * CY + col / CX;
+  cl.base = (cl.j * P.nx + cl.i) * P.nz;
   cl.live = false;
   cl.bad = false;
   cl.lo = cl.hi = 0;
@@ -530,7 +535,7 @@ __device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgc
 template <typename T>
 __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> &c, int &lo, int &hi,
                                           int &np, int &i, int &j) {
-  c.vc = bcast(src.c.vc, cc);
+  c.vcf = bcast(src.c.vcf, cc);
   c.uc0 = bcast(src.c.uc0, cc);
   c.duc = bcast(src.c.duc, cc);
   c.rho = bcast(src.c.rho, cc);
@@ -622,6 +627,8 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
   const int kf = tm.bz * kKT, kl = min(kf + kKT - 1, P.nz - 1);
   const long npix = long(P.nu) * P.nv;
   const long ntiles = gridDim.x;
+  prof[lane] = T(0);
+  prof[lane + 64] = T(0);
 
   for (int vv = 0; vv < views_per_block; ++vv) {
     const int view = view_base + blockIdx.y * views_per_block + vv;
@@ -670,8 +677,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       unsigned long long pend = __ballot(cl.live);
       T xa = T(0), xb = T(0);
       auto load_col = [&](int col) {
-        const int ij = bcast_i(cl.i | (cl.j << 16), col);
-        const long idx = (long(unsigned(ij) >> 16) * P.nx + (ij & 0xffff)) * P.nz + kc;
+        const unsigned idx = unsigned(bcast_i(cl.base, col) + kc);
         xa = vol0[idx];
         xb = two ? vol1[idx] : T(0);
       };
@@ -698,7 +704,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         act = kin && v.ul < v.ur;
         val = kin ? x * v.rho : T(0);
         if (use_prof) {
-          for (int u = lane; u < span; u += 64) prof[u] = T(0);
+          // prof[0, 128) is zero here (cleared at the start and after each
+          // outer product); longer spans clear the rest, spare entries included
+          for (int u = 128 + lane; u < span + W; u += 64) prof[u] = T(0);
           __builtin_amdgcn_wave_barrier();
           const bool fast = act && v.ul >= lo && v.ur <= hi && fits(v, np, P.nu);
           if (fast) {
@@ -721,6 +729,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           } else {
             for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
           }
+          __builtin_amdgcn_wave_barrier();
+          prof[lane] = T(0);
+          prof[lane + 64] = T(0);
           __builtin_amdgcn_wave_barrier();
         } else if (act) {
           T dummy = T(0);

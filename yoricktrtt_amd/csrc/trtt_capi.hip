@@ -173,6 +173,8 @@ void axial_analysis(trtt_hip_plan *p) {
   if (p->kind != trtt::CB3D && p->kind != trtt::PB3D) return;
   // the kernels broadcast detector and column indices packed in 16 bits
   if (p->nu > 65535 || p->nv > 65535 || p->nx > 65535 || p->ny > 65535) return;
+  // and address a phase of the volume with 32-bit offsets
+  if (double(p->nx) * p->ny * p->nz >= 2147483648.0) return;
   const bool cone = p->kind == trtt::CB3D;
   const double s = p->s_scl, hs0 = 0.5 * (p->s_deg + 1);
   const double hu_init = s / p->u_scl, hv_init = s / p->v_scl;
