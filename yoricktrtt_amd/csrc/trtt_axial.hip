@@ -37,17 +37,19 @@ constexpr int kKT = 64;  // k per tile (one per lane)
 // [0, 2], G(t) = 1/2 - (2-t)^4/24 + max(1-t, 0)^4/6 (the two pieces
 // 2t/3 - t^3/3 + t^4/8 on [0, 1] and 1/2 - (2-t)^4/24 on [1, 2] differ by
 // exactly (1-t)^4/6), so no per-piece select is needed.  The 1/24 and 1/6
-// are folded into the bases, s = (2-t)/24^(1/4) and r = max(1-t, 0)/6^(1/4),
-// so G = 1/2 - (s^2)^2 + (r^2)^2 is two FMAs on the squares: 8 instructions
-// with the clamp and the sign transfer.
+// are folded into the bases, s = max(2-t, 0)/24^(1/4) and
+// r = max(1-t, 0)/6^(1/4), so G = 1/2 - (s^2)^2 + (r^2)^2 is two FMAs on the
+// squares.  Both bases lie below 1, so the [0, 1] output clamp of the FMA
+// that forms them is the max(., 0), which also makes G = 1/2 for t > 2:
+// 7 instructions with the sign transfer (|x| is a source modifier).
 constexpr double kC24 = 0.45180100180492244;  // 24^(-1/4)
 constexpr double kC6 = 0.6389431042462724;    // 6^(-1/4)
 
 template <typename T>
 __device__ __forceinline__ T cdf3c(T x) {
-  const T t = fmin(fabs(x), T(2));
-  const T s = fma(t, T(-kC24), T(2.0 * kC24));
-  const T r = fmin(fmax(fma(t, T(-kC6), T(kC6)), T(0)), T(1));  // output clamp
+  const T t = fabs(x);
+  const T s = fmin(fmax(fma(t, T(-kC24), T(2.0 * kC24)), T(0)), T(1));
+  const T r = fmin(fmax(fma(t, T(-kC6), T(kC6)), T(0)), T(1));
   const T s2 = s * s, r2 = r * r;
   return copysign(fma(r2, r2, fma(s2, -s2, T(0.5))), x);
 }
