@@ -32,7 +32,7 @@ WINDOWS = {0: (12288, 16384, 24576), 1: (8192, 12288, 16384), 2: (24576, 32768)}
 VPBS = (0, 1, 2, 4)
 CHUNKS = (0, 2, 4)  # views per forward scratch round (0: as many as fit the budget)
 BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384),
-               3: (24576, 32768), 4: (16384, 24576), 5: (32768, 49152)}
+               3: (18432, 20480, 22528, 24576), 4: (14336, 16384, 24576), 5: (28672, 32768)}
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "yoricktrtt_amd", "tuning.json")
 
 
