@@ -813,61 +813,42 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
                                                        const int *__restrict__ list,
                                                        T *__restrict__ sino, int view_base,
                                                        int ntiles, int cap) {
-  __shared__ int hit_t[kThreads];
-  __shared__ int4 hit_r[kThreads];
-  __shared__ int nhit;
-  const int npu = (P.nu + kRedU - 1) / kRedU, npv = (P.nv + kRedV - 1) / kRedV;
-  const int pu = blockIdx.x % npu, pv = blockIdx.x / npu;
-  const int vloc = blockIdx.y;
   // 16 x 16 threads, each on 4 consecutive pixels of one row: windows start
   // on a multiple of 4 pixels and their rows are multiples of 4 long, so the
   // four pixels are inside or outside a window together and one 16-byte
-  // load reads them
+  // load reads them.  The block walks its list of overlapping windows (or
+  // every header when the list overflowed) with wave-uniform indices, read
+  // through scalar loads: no LDS, which the concurrent forward kernel needs.
   static_assert(sizeof(T) == 4, "four pixels per 16-byte load");
+  const int npu = (P.nu + kRedU - 1) / kRedU, npv = (P.nv + kRedV - 1) / kRedV;
+  const int pu = blockIdx.x % npu, pv = blockIdx.x / npu;
+  const int vloc = blockIdx.y;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int u = pu * kRedU + 4 * tx, v0 = pv * kRedV, v = v0 + ty;
-  const int u_lo = pu * kRedU, u_hi = u_lo + kRedU;
+  const int u = pu * kRedU + 4 * tx, v = pv * kRedV + ty;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int4 *h = hdr + long(vloc) * ntiles;
   const long b = long(vloc) * npu * npv + blockIdx.x;
   const int nlist = count[b];
   const bool binned = nlist <= kBinCap;
   const int total = binned ? nlist : ntiles;
-  for (int t0 = 0; t0 < total; t0 += kThreads) {
-    // stage up to 256 candidate tiles (the binned list, or every header)
-    // with their window in LDS, keeping those that overlap this block
-    if (threadIdx.x == 0) nhit = 0;
-    __syncthreads();
-    const int e = t0 + threadIdx.x;
-    if (e < total) {
-      const int t = binned ? list[b * kBinCap + e] : e;
-      const int4 r = h[t];
-      if (r.z != 0 && r.x < u_hi && r.x + r.z > u_lo && r.y < v0 + kRedV && r.y + r.w > v0) {
-        const int k = atomicAdd(&nhit, 1);
-        hit_t[k] = t;
-        hit_r[k] = r;
-      }
-    }
-    __syncthreads();
-    const int n = nhit;
-    // unconditional loads (threads outside a window read its first entries
-    // and add 0) so that the loads of several hits are in flight together
+  const int *lst = list + b * kBinCap;
+  const T *base = scratch + long(vloc) * ntiles * cap;
+  // unconditional loads (threads outside a window read its first entries
+  // and add 0) so that the loads of several windows are in flight together
 #pragma unroll 4
-    for (int q0 = 0; q0 < n; ++q0) {
-      const int tt = hit_t[q0];
-      const int4 r = hit_r[q0];
-      const T *src = scratch + (long(vloc) * ntiles + tt) * cap;
-      const int du = u - r.x, dv = v - r.y;
-      const bool in = du >= 0 && du < r.z && dv >= 0 && dv < r.w;
-      const float4 val = *reinterpret_cast<const float4 *>(src + (in ? dv * r.z + du : 0));
-      if (in) {
-        acc.x += val.x;
-        acc.y += val.y;
-        acc.z += val.z;
-        acc.w += val.w;
-      }
+  for (int e = 0; e < total; ++e) {
+    const int t = binned ? lst[e] : e;
+    const int4 r = h[t];
+    const T *src = base + long(t) * cap;
+    const int du = u - r.x, dv = v - r.y;
+    const bool in = r.z != 0 && du >= 0 && du < r.z && dv >= 0 && dv < r.w;
+    const float4 val = *reinterpret_cast<const float4 *>(src + (in ? dv * r.z + du : 0));
+    if (in) {
+      acc.x += val.x;
+      acc.y += val.y;
+      acc.z += val.z;
+      acc.w += val.w;
     }
-    __syncthreads();
   }
   if (v < P.nv) {
     T *row = sino + long(view_base + vloc) * P.nu * P.nv + long(v) * P.nu;
