@@ -140,22 +140,26 @@ struct Vox {
   double uc;
   T su, rho;
   float hs;      // half support in pixels
-  int ul, ur;
-  int ux;        // right bound before clamping at the detector edge
-  bool clipped;  // left bound clamped at the detector edge
+  int ul;        // first pixel, before clamping at the left detector edge
+  int ur;        // end pixel, clamped at the right detector edge
+  int ux;        // end pixel before clamping
 };
 
-// Pixel window [lo, hi) of the support [c - hs, c + hs] on a detector of n
-// pixels.  A support wholly past either edge gives lo >= hi (nothing to
-// add); float-to-int conversion saturates, and the right end is capped so
-// that hx + 1 cannot overflow.
-__device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int &hi, int &hx,
-                                         bool &clipped) {
+// Pixel window of the support [c - hs, c + hs] on a detector of n pixels:
+// first pixel lo (not clamped: it may be negative), end hi clamped at n and
+// hx unclamped.  Float-to-int conversion saturates, and both ends are capped
+// so that hx + 1 cannot overflow.
+__device__ __forceinline__ void clampw_f(float c, float hs, int n, int &lo, int &hi, int &hx) {
   const float l = c - hs, r = c + hs;
-  lo = max(int(floorf(l)), 0);
+  lo = int(floorf(fmaxf(l, -1.0e7f)));
   hx = int(floorf(fminf(r, 1.0e7f))) + 1;
   hi = min(hx, n);
-  clipped = l < 0.0f;
+}
+
+// The window meets the detector.
+template <typename T>
+__device__ __forceinline__ bool on_detector(const Vox<T> &v) {
+  return v.ur > max(v.ul, 0);
 }
 
 template <typename T>
@@ -168,19 +172,18 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
   v.su = frcp(hu);
   v.rho = c.rho * du;
   v.hs = float(T(P.hs0) * hu);
-  clampw_f(float(v.uc), v.hs, P.nu, v.ul, v.ur, v.ux, v.clipped);
+  clampw_f(float(v.uc), v.hs, P.nu, v.ul, v.ur, v.ux);
 }
 
 // Weights of pixels [ul, ul+NP) for a voxel whose unclamped window
 // [ul, ux) has at most NP pixels: NP - 1 CDF evaluations, and the last
 // weight is 1/2 - F because the CDF is exactly 1/2 from the footprint's right
-// end on.  An unclipped window starts at least 2h left of the centre, where
-// the CDF is exactly -1/2.  Entries past the window are zero.
+// end on.  The window starts (unclamped) at least 2h left of the centre,
+// where the CDF is exactly -1/2.  Entries past the window are zero.
 template <int NP, typename T>
 __device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[NP]) {
   const T x0 = T(double(v.ul) - v.uc) * v.su;
   T prev = T(-0.5);
-  if (v.clipped) prev = cdf3c<T>(x0);
 #pragma unroll
   for (int l = 0; l + 1 < NP; ++l) {
     const T nx = cdf3c<T>(x0 + T(l + 1) * v.su);
@@ -192,8 +195,7 @@ __device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[NP]) {
 
 // The weights of uweights<np> are exact on the detector when the unclamped
 // window fits np pixels, or when pixel ul + np - 1 lies past the right
-// detector edge (its weight is then never used; voxels wholly past that
-// edge have the one-pixel window [nu - 1, nu)).
+// detector edge (its weight is then never used).
 template <typename T>
 __device__ __forceinline__ bool fits(const Vox<T> &v, int np, int nu) {
   return v.ux - v.ul <= np || v.ul + np > nu;
@@ -350,12 +352,11 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
 }
 
 // Forward profile steps of one voxel whose window has at most NP pixels
-// (pp: the profile at the window's first pixel).  Unclipped windows start on
-// strictly monotone pixels across the lanes: plain accumulation over all NP
-// pixels (zero weights past the footprint; past a right detector edge they
-// land beyond the column span, which is never read, and the profile has
-// spare entries for them).  Windows clamped at the left detector edge may
-// share their first pixel: LDS atomics, issued after the plain steps.
+// (pp: the profile at the window's first pixel, unclamped: W margin entries
+// precede the profile, spare entries follow it, and neither is read).  The
+// pixels past the footprint get zero weights.  Where the host has shown that
+// consecutive voxels start their windows on strictly monotone pixels (pl,
+// wave-uniform), plain read-modify-write steps; otherwise LDS atomics.
 template <int NP, typename T>
 __device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, bool pl) {
   T w[NP];
@@ -366,12 +367,9 @@ __device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, boo
       pp[l] += w[l] * val;
       __builtin_amdgcn_wave_barrier();
     }
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (!pl) {
+  } else {
 #pragma unroll
-    for (int l = 0; l < NP; ++l)
-      if (l < v.ur - v.ul) atomicAdd(pp + l, w[l] * val);
+    for (int l = 0; l < NP; ++l) atomicAdd(pp + l, w[l] * val);
   }
 }
 
@@ -420,9 +418,10 @@ __device__ __forceinline__ T gather_n(int np, const Vox<T> &v, const T *qq) {
 template <bool FWD, typename T>
 __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T val, T *out,
                            const T *in, T &acc) {
-  const int npu = v.ur - v.ul, npv = c.vr - c.vl;
+  const int ul = max(v.ul, 0);  // the part of the window on the detector
+  const int npu = v.ur - ul, npv = c.vr - c.vl;
   if (npu <= 0 || npv <= 0) return;
-  const T xu0 = T(double(v.ul) - v.uc) * v.su;
+  const T xu0 = T(double(ul) - v.uc) * v.su;
   const T xv0 = -c.vcf * c.sv;
   T fv0 = cdf3c<T>(xv0);
   T sum = T(0);
@@ -435,7 +434,7 @@ __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T
       const T fu1 = cdf3c<T>(xu0 + T(l + 1) * v.su);
       const T w = wv * (fu1 - fu0);
       fu0 = fu1;
-      const long pix = long(c.vl + m) * P.nu + (v.ul + l);
+      const long pix = long(c.vl + m) * P.nu + (ul + l);
       if (FWD)
         unsafeAtomicAdd(out + pix, w * val);
       else
@@ -621,7 +620,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
   static_assert(NCOL <= 64, "one column per lane");
   const int lane = threadIdx.x;
   T *win = reinterpret_cast<T *>(g_smem);
-  T *prof = win + wcap;
+  T *prof = win + wcap + W;  // after W margin entries
   TileMap tm;
   tm.init(P, CX, CY);
   const int k = tm.bz * kKT + lane;
@@ -703,16 +702,18 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         Vox<T> v;
         bool act = false;
         voxel(P, a12, c, kc, v);
-        act = kin && v.ul < v.ur;
+        act = kin && on_detector(v);
         val = kin ? x * v.rho : T(0);
         if (use_prof) {
           // prof[0, 128) is zero here (cleared at the start and after each
           // outer product); longer spans clear the rest, spare entries included
           for (int u = 128 + lane; u < span + W; u += 64) prof[u] = T(0);
           __builtin_amdgcn_wave_barrier();
-          const bool fast = act && v.ul >= lo && v.ur <= hi && fits(v, np, P.nu);
+          // windows starting left of the detector use the W margin entries
+          // in front of the profile (never read)
+          const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
           if (fast) {
-            profile_n<W>(np, v, prof + (v.ul - lo), val, plain && !v.clipped);
+            profile_n<W>(np, v, prof + (v.ul - lo), val, plain != 0);
           } else if (act) {
             T dummy = T(0);
             slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
@@ -875,7 +876,9 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
   static_assert(NCOL <= 64, "one column per lane");
   T *region = reinterpret_cast<T *>(g_smem);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  T *qrow = region + lds_cap + wave * (pmax + W);  // W zero entries past each Q row
+  // Q row of the wave: W zero entries before and after it
+  T *qrow = region + lds_cap + wave * (pmax + 2 * W) + W;
+  if (lane < W) qrow[lane - W] = T(0);
   __shared__ int red[4 * kWaves];
   TileMap tm;
   tm.init(P, CX, CY);
@@ -974,8 +977,10 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               const bool kin = k < P.nz;
               Vox<T> v;
               voxel(P, a12, c, kin ? k : P.nz - 1, v);
-              if (kin && v.ul < v.ur) {
-                if (v.ul >= lo && v.ur <= hi && fits(v, np, P.nu)) {
+              if (kin && on_detector(v)) {
+                // windows starting left of the detector read the W zero
+                // entries in front of the Q row
+                if (v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu)) {
                   // all np pixels: zero weights past the footprint, zero Q
                   // past the column span (right detector edge)
                   contrib[q] = gather_n<W>(np, v, qrow + (v.ul - lo)) * v.rho;
@@ -992,7 +997,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               if (k < P.nz) {
                 Vox<T> v;
                 voxel(P, a12, c, k, v);
-                if (v.ul < v.ur) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+                if (on_detector(v)) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
               }
             }
           }
@@ -1104,9 +1109,9 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
   int npair = 0;
   if (job == 0) {
     const int wcap = tu.fwd_window_bytes / int(sizeof(T));
-    // window, profile with W spare entries; at least 128 entries past the
-    // window for the batched outer product of spans up to 128 pixels
-    const int lds_bytes = (wcap + max(pmax + W, 128)) * int(sizeof(T));
+    // window, W margin entries, profile with W spare entries (at least 128
+    // entries past the window for the batched outer product)
+    const int lds_bytes = (wcap + W + max(pmax + W, 128)) * int(sizeof(T));
     const unsigned ntiles = unsigned(axial_tiles(P, 0, tu));
     const int npt = axial_reduce_blocks(P);
     T *volk = static_cast<T *>(volk_buf);
@@ -1176,10 +1181,10 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     if (vpl == 2 && pmax2 <= 0) return hipErrorInvalidValue;
     const int pm = vpl == 2 ? pmax2 : pmax;
     const int lds_cap = tu.bwd_window_bytes / int(sizeof(T));
-    // window, one Q row with W zero entries per wave; at least 128 VPL
-    // entries past the window for the batched Q rows
+    // window, one Q row per wave with W zero entries on each side; at least
+    // 128 VPL entries past the window for the batched Q rows
     const int lds_bytes =
-        tu.bwd_window_bytes + max(axial::kWaves * (pm + W), 128 * vpl) * int(sizeof(T));
+        tu.bwd_window_bytes + max(axial::kWaves * (pm + 2 * W), 128 * vpl) * int(sizeof(T));
     dim3 grid(unsigned(axial_tiles(P, 1, tu)), nrows);
     const auto *i = static_cast<const T *>(in);
     auto *o = static_cast<T *>(out);
