@@ -709,14 +709,16 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           // outer product); longer spans clear the rest, spare entries included
           for (int u = 128 + lane; u < span + W; u += 64) prof[u] = T(0);
           __builtin_amdgcn_wave_barrier();
-          // windows starting left of the detector use the W margin entries
-          // in front of the profile (never read)
+          // Windows starting left of the detector use the W margin entries
+          // in front of the profile (never read).  Every lane runs the steps:
+          // lanes off the fast path add 0 into the margin, and the rare
+          // voxels that need the direct path take it in a separate block.
           const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
-          if (fast) {
-            profile_n<W>(np, v, prof + (v.ul - lo), val, plain != 0);
-          } else if (act) {
+          profile_n<W>(np, v, fast ? prof + (v.ul - lo) : prof - W, fast ? val : T(0),
+                       plain != 0);
+          if (__ballot(act && !fast)) {
             T dummy = T(0);
-            slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
+            if (act && !fast) slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
           }
           __builtin_amdgcn_wave_barrier();
           // outer product P x cv; the lanes sit on distinct window columns and
@@ -977,16 +979,18 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               const bool kin = k < P.nz;
               Vox<T> v;
               voxel(P, a12, c, kin ? k : P.nz - 1, v);
-              if (kin && on_detector(v)) {
-                // windows starting left of the detector read the W zero
-                // entries in front of the Q row
-                if (v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu)) {
-                  // all np pixels: zero weights past the footprint, zero Q
-                  // past the column span (right detector edge)
-                  contrib[q] = gather_n<W>(np, v, qrow + (v.ul - lo)) * v.rho;
-                } else {
-                  slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
-                }
+              // All np pixels: zero weights past the footprint, zero Q past
+              // the column span (right detector edge) and in the W entries
+              // in front of the row (windows starting left of the detector).
+              // Every lane gathers: lanes off the fast path read the zero
+              // entries, and the rare voxels that need the direct path take it
+              // in a separate block.
+              const bool act = kin && on_detector(v);
+              const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
+              const T gq = gather_n<W>(np, v, fast ? qrow + (v.ul - lo) : qrow - W);
+              contrib[q] = fast ? gq * v.rho : T(0);
+              if (__ballot(act && !fast)) {
+                if (act && !fast) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
               }
             }
             __builtin_amdgcn_wave_barrier();
