@@ -440,3 +440,35 @@ def test_axial_backward_tiles_vs_oracle(lib, oracle, nt):
         assert rel_err(bwd, rb) < 1e-5, tile
     P.close()
 
+
+
+def test_axial_direct_path_and_edges_vs_oracle(lib, oracle):
+    """The smallest LDS windows send most tiles down the direct path of both
+    kernels (the block entered when a lane's window does not fit), on a
+    detector smaller than the projected volume, so that windows cross both
+    edges (margin entries in front of the profile and the Q row)."""
+    from yoricktrtt_amd import autotune as AT
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("direct path and edges")
+    nx, ny, nz, nu, nv, V = 26, 23, 60, 52, 44, 5
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, np.linspace(0.3, 5.9, V), 110.0, 170.0,
+                        u_off=-2.5, v_off=1.5)
+    P = Projector(geo, model="cubic", precision="f32")
+    assert P.path > 0
+    rng = np.random.default_rng(17)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    rf = np.stack([oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0,
+                               geo.xos[v], geo.xsd[v], True, 0) for v in range(V)])
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v], True, 1)
+    for wf, wb in [(4096, 4096), (8192, 16384)]:
+        AT.set_tuning(P, fwd_window_bytes=wf, bwd_window_bytes=wb)
+        fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+        bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+        assert rel_err(fwd, rf) < 1e-5, wf
+        assert rel_err(bwd, rb) < 1e-5, wb
+    P.close()
