@@ -190,7 +190,9 @@ int trtt_hip_plan_path(const trtt_hip_plan_t *plan);
  * flight. */
 typedef struct trtt_hip_tuning {
   int fwd_tile;            /* 0: 8x4, 1: 4x4, 2: 8x8 voxel columns x 64 k */
-  int bwd_tile;            /* 0: 8x8, 1: 8x4, 2: 4x4 voxel columns x 64 k */
+  int bwd_tile;            /* 0: 8x8, 1: 8x4, 2: 4x4 voxel columns x 64 k;
+                              3: 8x4, 4: 4x4, 5: 8x8 x 128 k (plans whose column
+                              profiles over 128 slices exceed 256 pixels refuse them) */
   int fwd_window_bytes;    /* LDS detector window of a forward wave, 4096..65536 */
   int fwd_views_per_block; /* views per forward workgroup, 0 = from the grid size */
   int bwd_window_bytes;    /* LDS detector window of a backward workgroup, 4096..131072 */
