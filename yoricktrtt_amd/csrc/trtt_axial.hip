@@ -1055,7 +1055,6 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 // workgroup, backward four waves.
 constexpr int kFwdTiles[kNumFwdTiles][2] = {{8, 4}, {4, 4}, {8, 8}};
 constexpr int kBwdTiles[kNumBwdTiles][2] = {{8, 8}, {8, 4}, {4, 4}, {8, 4}, {4, 4}, {8, 8}};
-constexpr int kAxialPmax = 256;  // longest column profile held in LDS
 
 int axial_tiles(const DevPlan &P, int cx, int cy, int kt) {
   return ((P.nx + cx - 1) / cx) * ((P.ny + cy - 1) / cy) * ((P.nz + kt - 1) / kt);
