@@ -61,14 +61,15 @@ def phantom(seed=3, n_ell=30):
 
 
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "axial_fwd_cone256_traffic.txt")
+ISSUE_FILE = os.path.join(HERE, "profiles", "axial_cone256_issue_counters.txt")
 
 
-def measured_traffic(tuning, kernel):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a 1 GiB copy),
-    when it was measured with the launch shape in use; else None."""
+def profile_values(path, tuning):
+    """`name = value` lines of a committed rocprofv3 summary, or None when
+    the summary is absent or was measured with another launch shape than
+    `tuning` (its `# tuning:` line)."""
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(path).read()
     except OSError:
         return None
     vals, shape = {}, None
@@ -80,8 +81,31 @@ def measured_traffic(tuning, kernel):
             vals[k.strip()] = float(v)
     if not tuning or not shape or any(int(shape[k]) != int(tuning[k]) for k in shape):
         return None
+    return vals
+
+
+def measured_traffic(tuning, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a 1 GiB copy),
+    when it was measured with the launch shape in use; else None."""
+    vals = profile_values(TRAFFIC_FILE, tuning)
     key = "bwd_traffic_bytes_per_launch" if kernel == "axial::bwd" else "fwd_traffic_bytes_per_launch"
-    return int(vals[key]) if key in vals else None
+    return int(vals[key]) if vals and key in vals else None
+
+
+def measured_issue(tuning, kernel, launch_ms):
+    """VALU issue rate of the dominant kernel: its wave64 VALU instructions
+    per launch (committed SQ_INSTS_VALU summary, same launch shape) over the
+    launch time measured now, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+    (non-packed instructions); None without a matching summary."""
+    vals = profile_values(ISSUE_FILE, tuning)
+    key = "bwd_valu_insts_per_launch" if kernel == "axial::bwd" else "fwd_valu_insts_per_launch"
+    if not vals or key not in vals or launch_ms <= 0:
+        return None
+    rate = vals[key] / (launch_ms * 1e-3)
+    peak = vals["valu_issue_peak_per_s"]
+    return {"unit": "wave64 VALU instructions/s", "insts_per_launch": int(vals[key]),
+            "achieved": float(f"{rate:.4g}"), "peak": peak, "frac": round(rate / peak, 4)}
 
 
 def algorithmic_bytes(esz=4):
@@ -272,6 +296,8 @@ def main():
         },
     }
     result["roofline"]["traffic"] = measured_traffic(tuning, result["roofline"]["kernel"])
+    # the axial kernels are bound by instruction issue, not by HBM
+    result["roofline"]["issue"] = measured_issue(tuning, result["roofline"]["kernel"], dom_ms)
     if not args.no_cpu:
         print("bench: timing the reference C plugin on the host", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_reference_baseline(geo, args.cpu_threads)
