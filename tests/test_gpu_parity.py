@@ -2,7 +2,7 @@
 FP64 oracle.  Parity measure: max|gpu - ref| / max|ref| per output.
 
 Tolerances (written per test):
-  FP32 plans / drop-ins   1e-5   (north-star bound for FP32 projections)
+  FP32 plans / drop-ins   1e-5   (the FP32 projection tolerance)
   FP64 plans / drop-ins   1e-12  (table mode reproduces the reference's FP64
                                   coordinates and floor() indices)
   adjoint <Ax,y> = <x,A'y> with FP64 dot products: 1e-6 (FP32), 1e-12 (FP64)
