@@ -209,6 +209,36 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *plan, const trtt_hip_tuning_t *tun
 int trtt_hip_plan_set_timing(trtt_hip_plan_t *plan, int on);
 int trtt_hip_plan_kernel_time(trtt_hip_plan_t *plan, int job, double *ms, int *launches);
 
+/* Sparse system matrix of a 2-D plan (TRTT_HIP_PB2D, _FB2D or _LF2D; one
+ * phase, no time weights), built and kept on the device.  Replaces the
+ * matrix assembly of Ytrtt_2D_projectors.i:212-369 (_trtt_2D_sparse_coefs,
+ * _trtt_2D_sparse_matrix), whose per-pixel loop calls trtt_PB2D_sparser /
+ * trtt_FB2D_sparser (plugin/trtt_2D_projectors.c:393-536) or lf2d_sparser
+ * (tomography-LongFessler-model/LongFessler2D.c:472-571), and the products
+ * with the matrix (trtt_mvmult, plugin/trtt_common.c:62-69).
+ *   row (view, bin) = view * nv + bin, column (pixel) = j * nx + i;
+ *   entries: the non-zero coefficients of the plan's own footprint model,
+ *   in the order view, pixel, bin (the reference's order);
+ *   decimals  6: each coefficient rounded to 6 decimals, half to even, as
+ *             the reference's "%f" text round trip leaves it;
+ *            -1: exact coefficients.
+ * Limits: views <= 65535, views x pixels < 2^31, entries < 2^31 - 512.
+ * Synchronous; the matrix keeps its own device memory (COO, CSR and CSC:
+ * entries x (3 x value size + 16) bytes). */
+typedef struct trtt_hip_sparse trtt_hip_sparse_t;
+trtt_hip_sparse_t *trtt_hip_sparse_create(trtt_hip_plan_t *plan, int decimals);
+/* Number of entries, -1 for a null matrix. */
+long trtt_hip_sparse_nnz(const trtt_hip_sparse_t *m);
+/* Entries in the reference's order with 1-based indices, as the
+ * MCops_coefs / MCops_row_indices / MCops_col_indices arrays of
+ * Ytrtt_2D_projectors.i:326-350 (host arrays of nnz entries).  Synchronous. */
+int trtt_hip_sparse_coo(const trtt_hip_sparse_t *m, double *coefs, long *rows, long *cols);
+/* sino += M img; img += M' sino (device pointers, plan precision; img is
+ * [ny][nx], sino [nviews][nv]).  Asynchronous on `stream`. */
+int trtt_hip_sparse_forward(trtt_hip_sparse_t *m, const void *img, void *sino, void *stream);
+int trtt_hip_sparse_backward(trtt_hip_sparse_t *m, const void *sino, void *img, void *stream);
+int trtt_hip_sparse_destroy(trtt_hip_sparse_t *m);
+
 long trtt_hip_plan_nviews(const trtt_hip_plan_t *plan);
 long trtt_hip_plan_volume_size(const trtt_hip_plan_t *plan);   /* nt*nz*ny*nx */
 long trtt_hip_plan_sino_size(const trtt_hip_plan_t *plan);     /* nviews*nv*nu */

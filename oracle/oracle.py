@@ -132,6 +132,29 @@ class Oracle:
         return dst
 
     @staticmethod
+    def sparse2d_coo(column, nviews, nx, ny, decimals=6):
+        """Sparse system matrix assembled as Ytrtt_2D_projectors.i:252-290
+        and :326-350 do it: for every view, pixel (j, then i) and bin, the
+        non-zero coefficients of ``column(view, i, j)`` (the nv coefficients
+        of one pixel, e.g. a projection of a one-hot image), written as "%f"
+        text and read back when ``decimals`` is 6.  Rows (view, bin) and
+        columns j nx + i are 1-based.  Pure Python: small cases only."""
+        coefs, rows, cols = [], [], []
+        for v in range(nviews):
+            for j in range(ny):
+                for i in range(nx):
+                    c = np.asarray(column(v, i, j), dtype=np.float64)
+                    nz = np.nonzero(c != 0.0)[0]
+                    vals = c[nz]
+                    if decimals == 6:
+                        vals = np.array([float("%f" % x) for x in vals])
+                    coefs.append(vals)
+                    rows.append(v * c.size + nz + 1)
+                    cols.append(np.full(nz.size, j * nx + i + 1))
+        cat = (lambda a, t: np.concatenate(a).astype(t) if a else np.zeros(0, t))
+        return cat(coefs, np.float64), cat(rows, np.int64), cat(cols, np.int64)
+
+    @staticmethod
     def _fp(table, kind):
         if table is None:
             return FP_CUBIC, np.zeros(1), 1, 1.0, 0.0

@@ -33,6 +33,11 @@ def golden2d():
 
 
 @pytest.fixture(scope="session")
+def golden_sparse2d():
+    return dict(np.load(os.path.join(GOLDEN, "sparse2d.npz")))
+
+
+@pytest.fixture(scope="session")
 def golden_xform():
     return dict(np.load(os.path.join(GOLDEN, "xform.npz")))
 

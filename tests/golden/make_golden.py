@@ -8,13 +8,16 @@ and records inputs and outputs of every projector on small seeded cases:
 * ``footprint.npz`` reference ``_trtt_cumsum`` of cubic B-spline samples
 * ``proj3d.npz``    trtt_{PB,CB}3D[_cubic], job 0 and 1, four geometries
 * ``proj2d.npz``    trtt_{PB,FB}2D[_cubic], job 0 and 1, three geometries
+* ``sparse2d.npz``  trtt_{PB,FB}2D_sparser: the coefficient column of every
+                    pixel on the views of the 2-D geometries
 
 The per-view XOS/XSD pairs are composed with the reference transform code in
 the same sequence as the Yorick system builders
 (Tomography_3D/Ytrtt3D_system.i:436-480, Tomography_2D/Ytrtt2D_system.i:529-589,
 Ytrtt_3D_projectors.i:104-110), so the fixtures pin the host geometry too.
 
-Usage:  make -C oracle && python tests/golden/make_golden.py
+Usage:  make -C oracle && python tests/golden/make_golden.py [fixture ...]
+(default: every fixture)
 """
 from __future__ import annotations
 
@@ -92,6 +95,10 @@ def load():
     cub3 = [dp, dp, L, L, L, d, I, L, L, d, d, dp, dp, I]
     tab2 = [dp, dp, L, L, d, I, dp, dp, L, d, dp, L, d, dp, dp, I]
     cub2 = [dp, dp, L, L, d, I, L, d, dp, dp, I]
+    sparser = [dp, L, L, L, L, d, I, dp, dp, L, d, dp, L, d, dp, dp]
+    for n in ("trtt_PB2D_sparser", "trtt_FB2D_sparser"):
+        getattr(lib, n).argtypes = sparser
+        getattr(lib, n).restype = I
     for n, a in (("trtt_PB3D", tab3), ("trtt_CB3D", tab3), ("trtt_PB3D_cubic", cub3),
                  ("trtt_CB3D_cubic", cub3), ("trtt_PB2D", tab2), ("trtt_FB2D", tab2),
                  ("trtt_PB2D_cubic", cub2), ("trtt_FB2D_cubic", cub2)):
@@ -317,21 +324,57 @@ def proj2d_cases(lib):
     return out
 
 
-def main():
+def sparse2d_cases(lib, proj2d):
+    """Reference sparser columns on the geometries of proj2d.npz (at most
+    four views each): col[v, j, i] = trtt_{PB,FB}2D_sparser(i, j) on view v."""
+    g = proj2d
+    ny, nx = g["img"].shape
+    s = float(g["s"])
+    fp, cum, coord, step = ref_table(lib, 1000)
+    out = {"nx": np.array(nx), "ny": np.array(ny), "s": np.array(s)}
+    for name in ("fan_a", "fan_edge", "par_a", "par_edge"):
+        nv, vs = int(g[f"{name}/nv"]), float(g[f"{name}/vs"])
+        xos, xsd = g[f"{name}/xos"][:4], g[f"{name}/xsd"][:4]
+        fn = getattr(lib, "trtt_FB2D_sparser" if name.startswith("fan") else "trtt_PB2D_sparser")
+        cols = np.zeros((xos.shape[0], ny, nx, nv))
+        for v in range(xos.shape[0]):
+            xo, xs = np.ascontiguousarray(xos[v]), np.ascontiguousarray(xsd[v])
+            for j in range(ny):
+                for i in range(nx):
+                    o = cols[v, j, i]
+                    assert fn(_p(o), i, j, nx, ny, s, 3, _p(fp), _p(cum), len(cum), step,
+                              _p(coord), nv, vs, _p(xo), _p(xs)) == 0
+        out[f"{name}/xos"] = xos
+        out[f"{name}/xsd"] = xsd
+        out[f"{name}/nv"] = np.array(nv)
+        out[f"{name}/vs"] = np.array(vs)
+        out[f"{name}/cols"] = cols
+    return out
+
+
+def main(names=None):
     if not os.path.exists(LIB):
         sys.exit(f"{LIB} missing: run `make -C oracle` where the reference sources exist")
     lib = load()
-    np.savez_compressed(os.path.join(HERE, "xform.npz"), **xform_cases(lib))
-    fp1, cum1, coord1, step1 = ref_table(lib, 1000)
-    fp2, cum2, coord2, step2 = ref_table(lib, 100000)
-    np.savez_compressed(os.path.join(HERE, "footprint.npz"), cum1000=cum1, step1000=np.array(step1),
-                        cum100000_sub=cum2[::997], cum100000_last=np.array(cum2[-1]),
-                        step100000=np.array(step2))
-    np.savez_compressed(os.path.join(HERE, "proj3d.npz"), **proj3d_cases(lib))
-    np.savez_compressed(os.path.join(HERE, "proj2d.npz"), **proj2d_cases(lib))
-    for f in ("xform", "footprint", "proj3d", "proj2d"):
+    names = names or ["xform", "footprint", "proj3d", "proj2d", "sparse2d"]
+    if "xform" in names:
+        np.savez_compressed(os.path.join(HERE, "xform.npz"), **xform_cases(lib))
+    if "footprint" in names:
+        fp1, cum1, coord1, step1 = ref_table(lib, 1000)
+        fp2, cum2, coord2, step2 = ref_table(lib, 100000)
+        np.savez_compressed(os.path.join(HERE, "footprint.npz"), cum1000=cum1,
+                            step1000=np.array(step1), cum100000_sub=cum2[::997],
+                            cum100000_last=np.array(cum2[-1]), step100000=np.array(step2))
+    if "proj3d" in names:
+        np.savez_compressed(os.path.join(HERE, "proj3d.npz"), **proj3d_cases(lib))
+    if "proj2d" in names:
+        np.savez_compressed(os.path.join(HERE, "proj2d.npz"), **proj2d_cases(lib))
+    if "sparse2d" in names:
+        proj2d = np.load(os.path.join(HERE, "proj2d.npz"))
+        np.savez_compressed(os.path.join(HERE, "sparse2d.npz"), **sparse2d_cases(lib, proj2d))
+    for f in names:
         print(f, os.path.getsize(os.path.join(HERE, f + ".npz")), "bytes")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -71,6 +71,12 @@ _SIGS = {
     "trtt_hip_plan_volume_size": ([vp], L),
     "trtt_hip_plan_sino_size": ([vp], L),
     "trtt_hip_plan_destroy": ([vp], I),
+    "trtt_hip_sparse_create": ([vp, I], vp),
+    "trtt_hip_sparse_nnz": ([vp], L),
+    "trtt_hip_sparse_coo": ([vp, dp, lp, lp], I),
+    "trtt_hip_sparse_forward": ([vp, vp, vp, vp], I),
+    "trtt_hip_sparse_backward": ([vp, vp, vp, vp], I),
+    "trtt_hip_sparse_destroy": ([vp], I),
     "trtt_hip_tv": ([vp, vp, vp, vp, L, L, L, L, D, D, D, I, vp], I),
     "trtt_hip_bspline_filter": ([vp, vp, L, L, L, L, I, I, vp], I),
     "trtt_unet_conv_fwd": ([vp, vp, vp, vp, vp, I, I, I, I, I, I, I, I, I, vp], I),

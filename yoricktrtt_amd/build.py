@@ -8,7 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrtt_hip.so")
-SOURCES = ["trtt_kernels.hip", "trtt_axial.hip", "trtt_regul.hip", "trtt_unet.hip", "trtt_capi.hip",
+SOURCES = ["trtt_kernels.hip", "trtt_axial.hip", "trtt_regul.hip", "trtt_unet.hip", "trtt_sparse.hip",
+           "trtt_capi.hip",
            "trtt_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -17,7 +17,8 @@ and results.  Configuration 5 (the learned warm start) lives in
    (seed 2) of 0.02 mm^-1 per unit; analytic fan-beam data with Poisson
    noise at 1e5 photons and inverse-variance weights; 50 iterations; then the
    Long & Fessler SF projector on the same geometry (timing, adjoint test
-   and its own 50 iterations).
+   and its own 50 iterations), and the sparse-matrix mode (matrix built on
+   the GPU from the tabulated footprint, products timed, its own solve).
 3. 3-D cone beam, 256^3 voxels of 1 mm, 360 views over 360 deg, 512 (v,
    transaxial) x 384 (u, axial) flat detector at 1 mm, Rsc = 1000 mm,
    Rsd = 1500 mm, FP32; 30 random ellipsoids (seed 3); data = this
@@ -219,7 +220,49 @@ def config2(iters=50, tv_weight=2.0, tv_eps=0.01, photons=1e5, mu_scale=0.02):
     lf.close()
     out["lf2d"] = dict(fwd_ms=1e3 * lf_fwd, bwd_ms=1e3 * lf_bwd,
                        adjoint_rel=abs(lhs - rhs) / abs(lhs), **sol2)
+    out["sparse"] = _config2_sparse(geo, truth, y, w, tv, sample, iters)
     return out
+
+
+def _config2_sparse(geo, truth, y, w, tv, sample, iters):
+    """The reference's sparse-matrix mode on the configuration-2 geometry:
+    tabulated footprint, coefficients rounded to 6 decimals
+    (Ytrtt_2D_projectors.i:212-369), the matrix built on the GPU, the two
+    products timed against the table projector, then the same solve."""
+    from .projectors import Projector
+    from .reconstruction import DataTerm, Objective
+
+    torch = _torch()
+    dev = truth.device
+    Pt = Projector(geo, model="table", precision="f32")
+    t0 = _sync()
+    M = Pt.sparse_matrix(6)
+    build = _sync() - t0
+    sino = torch.zeros(Pt.sino_shape, dtype=torch.float32, device=dev)
+    fwd, bwd = _projection_times(M, truth, sino)
+    pfwd, pbwd = _projection_times(Pt, truth, sino)
+    yr = torch.rand(Pt.sino_shape, generator=torch.Generator(device=dev).manual_seed(9), device=dev)
+    mx, px = M.forward(truth), Pt.forward(truth)
+    my, py = M.backward(yr), Pt.backward(yr)
+    rel = (float((mx - px).abs().max() / px.abs().max()),
+           float((my - py).abs().max() / py.abs().max()))
+    # bytes per product: every entry (FP32 value + 32-bit index), the pointers,
+    # the gathered vector once and the output read and written
+    nnz, nrows, ncols = M.nnz, geo.nviews * geo.nv, geo.nx * geo.ny
+    b_fwd = nnz * 8 + (nrows + 1) * 4 + ncols * 4 + nrows * 8
+    b_bwd = nnz * 8 + (ncols + 1) * 4 + nrows * 4 + ncols * 8
+    D = DataTerm(M, y, w)
+    n = geo.nx
+    D.capture(torch.zeros((n, n), dtype=torch.float32, device=dev),
+              torch.zeros((n, n), dtype=torch.float32, device=dev))
+    _, sol = _solve(Objective(D, [tv]), torch.zeros((n, n), dtype=torch.float32, device=dev),
+                    iters, truth, sample)
+    M.close()
+    Pt.close()
+    return dict(nnz=nnz, build_s=build, fwd_ms=1e3 * fwd, bwd_ms=1e3 * bwd,
+                fwd_GBps=b_fwd / fwd / 1e9, bwd_GBps=b_bwd / bwd / 1e9,
+                table_projector_fwd_ms=1e3 * pfwd, table_projector_bwd_ms=1e3 * pbwd,
+                rel_vs_table_projector=rel, **sol)
 
 
 def cone_setup(n=256, nviews=360, nu=384, nv=512, pitch=1.0, Rsc=1000.0, Rsd=1500.0):

@@ -1,6 +1,7 @@
 // C ABI of libtrtt_hip.so: plans (device-resident batched projectors) and the
 // reference plugin symbols (synchronous per-view drop-ins on host FP64 data).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,6 +27,18 @@ hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, c
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu);
 size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
 size_t axial_hdr_bytes(const DevPlan &P, int chunk, const AxialTuning &tu);
+hipError_t launch_sparse_columns(int kind, bool table, bool quirk, bool f64, const DevPlan &P,
+                                 bool fill, int *count, const long long *offset, void *val,
+                                 int *row, int *col, int round6, hipStream_t st);
+hipError_t sparse_scan(const int *count, long long *offset, long n, void *temp,
+                       size_t &temp_bytes, hipStream_t st);
+hipError_t sparse_sort(const unsigned *key, unsigned *key_out, unsigned *ids, unsigned *perm,
+                       long n, int bits, void *temp, size_t &temp_bytes, hipStream_t st);
+hipError_t sparse_compress(bool f64, const unsigned *key_sorted, const unsigned *perm,
+                           const void *val, const int *other, void *out_val, int *out_idx,
+                           int *ptr, long n, int n_major, hipStream_t st);
+hipError_t sparse_spmv(bool f64, const int *ptr, const int *idx, const void *val, const void *x,
+                       void *y, int n_major, hipStream_t st);
 }  // namespace trtt
 
 using trtt::ViewConst;
@@ -798,6 +811,259 @@ int trtt_hip_plan_destroy(trtt_hip_plan_t *p) {
   release(p);
   return TRTT_SUCCESS;
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Part 2b -- sparse system matrix of a 2-D plan
+// ===========================================================================
+struct trtt_hip_sparse {
+  int device = 0;
+  bool f64 = false;
+  long nnz = 0;
+  int nrows = 0, ncols = 0;
+  void *coo_val = nullptr;  // reference order: view, pixel, bin
+  int *coo_row = nullptr, *coo_col = nullptr;
+  int *csr_ptr = nullptr, *csr_col = nullptr;  // rows (view, bin)
+  void *csr_val = nullptr;
+  int *csc_ptr = nullptr, *csc_row = nullptr;  // columns (pixels)
+  void *csc_val = nullptr;
+};
+
+namespace {
+
+// Device buffer freed at scope exit.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  void reset() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  bool ensure(size_t n) {
+    if (n <= bytes && p) return true;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (!hip_ok(hipMalloc(&p, std::max<size_t>(n, 256)), "hipMalloc")) return false;
+    bytes = std::max<size_t>(n, 256);
+    return true;
+  }
+  template <typename U>
+  U *as() const {
+    return static_cast<U *>(p);
+  }
+};
+
+void sparse_release(trtt_hip_sparse *m) {
+  if (!m) return;
+  hipSetDevice(m->device);
+  for (void *q : {m->coo_val, static_cast<void *>(m->coo_row), static_cast<void *>(m->coo_col),
+                  static_cast<void *>(m->csr_ptr), static_cast<void *>(m->csr_col), m->csr_val,
+                  static_cast<void *>(m->csc_ptr), static_cast<void *>(m->csc_row), m->csc_val})
+    if (q) hipFree(q);
+  delete m;
+}
+
+int key_bits(long n) {
+  int b = 1;
+  while (b < 31 && (1L << b) < n) ++b;
+  return b;
+}
+
+template <typename U>
+bool dev_alloc(U *&p, size_t count) {
+  void *q = nullptr;
+  if (!hip_ok(hipMalloc(&q, std::max<size_t>(count * sizeof(U), 256)), "hipMalloc")) return false;
+  p = static_cast<U *>(q);
+  return true;
+}
+
+// Stable sort of the entries by `key` and the compressed form over n_major
+// keys: other index and value of every entry in key order, and pointers.
+bool compress_by(trtt_hip_sparse *m, const int *key, const int *other, int n_major, int *&ptr,
+                 int *&idx, void *&val, DevBuf &temp, hipStream_t st) {
+  const size_t esz = m->f64 ? 8 : 4;
+  const long n = m->nnz;
+  if (!dev_alloc(ptr, size_t(n_major) + 1) || !dev_alloc(idx, size_t(n))) return false;
+  if (!hip_ok(hipMalloc(&val, std::max<size_t>(esz * n, 256)), "hipMalloc")) return false;
+  DevBuf keys, ids, perm;
+  if (n > 0) {
+    if (!keys.ensure(4 * n) || !ids.ensure(4 * n) || !perm.ensure(4 * n)) return false;
+    const unsigned *k = reinterpret_cast<const unsigned *>(key);
+    size_t tb = 0;
+    if (!hip_ok(trtt::sparse_sort(k, keys.as<unsigned>(), ids.as<unsigned>(), perm.as<unsigned>(),
+                                  n, key_bits(n_major), nullptr, tb, st),
+                "radix sort"))
+      return false;
+    if (!temp.ensure(tb)) return false;
+    tb = temp.bytes;
+    if (!hip_ok(trtt::sparse_sort(k, keys.as<unsigned>(), ids.as<unsigned>(), perm.as<unsigned>(),
+                                  n, key_bits(n_major), temp.p, tb, st),
+                "radix sort"))
+      return false;
+  }
+  return hip_ok(trtt::sparse_compress(m->f64, keys.as<unsigned>(), perm.as<unsigned>(),
+                                      m->coo_val, other, val, idx, ptr, n, n_major, st),
+                "sparse compress") &&
+         hip_ok(hipStreamSynchronize(st), "sparse assembly");
+}
+
+}  // namespace
+
+extern "C" {
+
+trtt_hip_sparse_t *trtt_hip_sparse_create(trtt_hip_plan_t *p, int decimals) {
+  if (!p) {
+    fail("null argument");
+    return nullptr;
+  }
+  if (p->kind != trtt::PB2D && p->kind != trtt::FB2D && p->kind != trtt::LF2D) {
+    fail("sparse matrices are built for 2-D plans (PB2D, FB2D, LF2D)");
+    return nullptr;
+  }
+  if (p->nt != 1) {
+    fail("sparse matrices are built for plans of one phase");
+    return nullptr;
+  }
+  for (const ViewConst &g : p->h_views)
+    if (g.w0 != 1.0 || g.w1 != 0.0) {
+      fail("sparse matrices are built for plans without time interpolation weights");
+      return nullptr;
+    }
+  if (decimals != 6 && decimals != -1) {
+    fail("decimals must be 6 (coefficients as the reference's text round trip leaves them) or -1");
+    return nullptr;
+  }
+  const long npix = p->nx * p->ny, n = npix * p->nviews;
+  if (p->nviews > 65535 || n >= long(INT_MAX) || p->nviews * p->nv >= long(INT_MAX)) {
+    fail("sparse matrix too large (views <= 65535, views x pixels and views x bins < 2^31)");
+    return nullptr;
+  }
+  if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return nullptr;
+  hipStream_t st = nullptr;
+  const bool f64 = p->precision == TRTT_HIP_F64;
+  const bool tab = p->table, quirk = p->quirk_last_bin;
+  DevBuf count, offset, temp;
+  if (!count.ensure(4 * size_t(n)) || !offset.ensure(8 * size_t(n))) return nullptr;
+  if (!hip_ok(hipMemsetAsync(p->d_err, 0, sizeof(int), st), "hipMemset") ||
+      !hip_ok(trtt::launch_sparse_columns(p->kind, tab, quirk, f64, p->dev, false, count.as<int>(),
+                                          nullptr, nullptr, nullptr, nullptr, 0, st),
+              "sparse count"))
+    return nullptr;
+  size_t tb = 0;
+  if (!hip_ok(trtt::sparse_scan(count.as<int>(), offset.as<long long>(), n, nullptr, tb, st),
+              "scan") ||
+      !temp.ensure(tb))
+    return nullptr;
+  tb = temp.bytes;
+  if (!hip_ok(trtt::sparse_scan(count.as<int>(), offset.as<long long>(), n, temp.p, tb, st),
+              "scan"))
+    return nullptr;
+  long long last_off = 0;
+  int last_cnt = 0, err = 0;
+  if (!hip_ok(hipMemcpy(&last_off, offset.as<long long>() + (n - 1), 8, hipMemcpyDeviceToHost),
+              "hipMemcpy") ||
+      !hip_ok(hipMemcpy(&last_cnt, count.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost),
+              "hipMemcpy") ||
+      !hip_ok(hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy"))
+    return nullptr;
+  if (err) {
+    fail("IT WAS SUPPOSED TO BE HIGHLY IMPOSSIBLE: a pixel lies in the source plane");
+    return nullptr;
+  }
+  const long long nnz = last_off + last_cnt;
+  if (nnz > static_cast<long long>(INT_MAX) - 512) {
+    fail("sparse matrix has too many entries (at most 2^31 - 512)");
+    return nullptr;
+  }
+  trtt_hip_sparse *m = new trtt_hip_sparse;
+  m->device = p->device;
+  m->f64 = f64;
+  m->nnz = long(nnz);
+  m->nrows = int(p->nviews * p->nv);
+  m->ncols = int(npix);
+  const size_t esz = f64 ? 8 : 4;
+  if (!hip_ok(hipMalloc(&m->coo_val, std::max<size_t>(esz * m->nnz, 256)), "hipMalloc") ||
+      !dev_alloc(m->coo_row, size_t(m->nnz)) || !dev_alloc(m->coo_col, size_t(m->nnz)) ||
+      !hip_ok(trtt::launch_sparse_columns(p->kind, tab, quirk, f64, p->dev, true, nullptr,
+                                          offset.as<long long>(), m->coo_val, m->coo_row,
+                                          m->coo_col, decimals == 6 ? 1 : 0, st),
+              "sparse fill") ||
+      !hip_ok(hipStreamSynchronize(st), "sparse fill")) {
+    sparse_release(m);
+    return nullptr;
+  }
+  count.reset();
+  if (!compress_by(m, m->coo_row, m->coo_col, m->nrows, m->csr_ptr, m->csr_col, m->csr_val, temp,
+                   st) ||
+      !compress_by(m, m->coo_col, m->coo_row, m->ncols, m->csc_ptr, m->csc_row, m->csc_val, temp,
+                   st)) {
+    sparse_release(m);
+    return nullptr;
+  }
+  return m;
+}
+
+long trtt_hip_sparse_nnz(const trtt_hip_sparse_t *m) { return m ? m->nnz : -1; }
+
+int trtt_hip_sparse_coo(const trtt_hip_sparse_t *m, double *coefs, long *rows, long *cols) {
+  if (!m || !coefs || !rows || !cols) return fail("null argument");
+  if (!hip_ok(hipSetDevice(m->device), "hipSetDevice")) return TRTT_FAILURE;
+  const long n = m->nnz;
+  std::vector<int> r(n), c(n);
+  if (m->f64) {
+    if (!hip_ok(hipMemcpy(coefs, m->coo_val, 8 * n, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return TRTT_FAILURE;
+  } else {
+    std::vector<float> v(n);
+    if (!hip_ok(hipMemcpy(v.data(), m->coo_val, 4 * n, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return TRTT_FAILURE;
+    for (long e = 0; e < n; ++e) coefs[e] = v[e];
+  }
+  if (!hip_ok(hipMemcpy(r.data(), m->coo_row, 4 * n, hipMemcpyDeviceToHost), "hipMemcpy") ||
+      !hip_ok(hipMemcpy(c.data(), m->coo_col, 4 * n, hipMemcpyDeviceToHost), "hipMemcpy"))
+    return TRTT_FAILURE;
+  for (long e = 0; e < n; ++e) {
+    rows[e] = long(r[e]) + 1;
+    cols[e] = long(c[e]) + 1;
+  }
+  return TRTT_SUCCESS;
+}
+
+int trtt_hip_sparse_forward(trtt_hip_sparse_t *m, const void *img, void *sino, void *stream) {
+  if (!m || !img || !sino) return fail("null argument");
+  return hip_ok(trtt::sparse_spmv(m->f64, m->csr_ptr, m->csr_col, m->csr_val, img, sino,
+                                  m->nrows, static_cast<hipStream_t>(stream)),
+                "sparse forward")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_sparse_backward(trtt_hip_sparse_t *m, const void *sino, void *img, void *stream) {
+  if (!m || !img || !sino) return fail("null argument");
+  return hip_ok(trtt::sparse_spmv(m->f64, m->csc_ptr, m->csc_row, m->csc_val, sino, img,
+                                  m->ncols, static_cast<hipStream_t>(stream)),
+                "sparse backward")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_sparse_destroy(trtt_hip_sparse_t *m) {
+  sparse_release(m);
+  return TRTT_SUCCESS;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 // ===========================================================================
 // Part 1 -- reference plugin symbols

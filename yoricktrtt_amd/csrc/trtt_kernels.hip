@@ -648,6 +648,122 @@ __global__ __launch_bounds__(kThreads) void weighted_residual(T *__restrict__ s,
 }
 
 // ---------------------------------------------------------------------------
+// Sparse system matrix of a 2-D plan.  The coefficients of pixel p = j nx + i
+// on view v are what its footprint burns into the view's bins for a unit
+// value (trtt_PB2D_sparser / trtt_FB2D_sparser, trtt_2D_projectors.c:393-536;
+// lf2d_sparser, LongFessler2D.c:472-571), evaluated with the expressions of
+// the plan's own forward kernel.  Only non-zero coefficients are entries
+// (Ytrtt_2D_projectors.i:268-279), in the order view, pixel, bin.  One thread
+// per (view, pixel): FILL = false writes the entry count, FILL = true writes
+// the entries from the exclusive scan of the counts.
+// ---------------------------------------------------------------------------
+
+// x rounded to 6 decimals, half to even, as the reference's "%f" text round
+// trip of the coefficients (Ytrtt_2D_projectors.i:268-290) leaves it: the
+// exact product x 1e6 = t + e (FMA residual) decides the ties.
+__device__ __forceinline__ double round_decimals6(double x) {
+  const double t = x * 1.0e6;
+  const double e = fma(x, 1.0e6, -t);
+  double k = rint(t);
+  const double d = t - k;  // exact
+  if (d == 0.5 && e > 0.0) k += 1.0;
+  if (d == -0.5 && e < 0.0) k -= 1.0;
+  return k / 1.0e6;
+}
+
+template <typename T>
+__device__ __forceinline__ void sparse_entry(bool fill, T c, int round6, long long &o, int &n,
+                                             T *val, int *row, int *col, int r, int p) {
+  if (c == T(0)) return;
+  if (fill) {
+    val[o] = round6 ? T(round_decimals6(double(c))) : c;
+    row[o] = r;
+    col[o] = p;
+    ++o;
+  } else {
+    ++n;
+  }
+}
+
+template <typename T, int KIND, bool TABLE, bool QUIRK, bool FILL>
+__global__ __launch_bounds__(kThreads) void sparse_columns(DevPlan P, int *__restrict__ count,
+                                                           const long long *__restrict__ offset,
+                                                           T *__restrict__ val, int *__restrict__ row,
+                                                           int *__restrict__ col, int round6) {
+  const long npix = long(P.nx) * P.ny;
+  const long p = long(blockIdx.x) * kThreads + threadIdx.x;
+  const int view = blockIdx.y;
+  if (p >= npix) return;
+  const int i = int(p % P.nx), j = int(p / P.nx);
+  const long q = long(view) * npix + p;
+  const ViewConst &g = P.views[view];
+  long long o = FILL ? offset[q] : 0;
+  int n = 0;
+  if constexpr (KIND == LF2D) {
+    LFFoot f;
+    lf_footprint(P, g, i, j, f);
+    for (int k = f.kf; k <= f.kl; ++k) {
+      // lf_forward's coefficient for a unit value
+      const T c = T(lf_gamma(f, k, P.lf_size) * f.amp);
+      sparse_entry(FILL, c, round6, o, n, val, row, col, view * P.nv + k, int(p));
+    }
+  } else {
+    Foot f;
+    if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, i, j, 0, f)) {
+      if (!FILL) atomicOr(P.err, 1);
+    } else if (f.vl < f.vr) {
+      // fp_forward's coefficient for a unit value: weight x rho0
+      AxisFoot<T, TABLE, true> av;
+      av.init(P, f.vl, f.vc, f.hv, f.sv);
+      const T amp = T(f.rho0);
+      for (int mm = 0; mm < f.vr - f.vl; ++mm) {
+        const T c = av.next(mm) * amp;
+        sparse_entry(FILL, c, round6, o, n, val, row, col, view * P.nv + f.vl + mm, int(p));
+      }
+    }
+  }
+  if (!FILL) count[q] = n;
+}
+
+template <typename T, bool FILL>
+hipError_t launch_sparse_fill(int kind, bool table, bool quirk, const DevPlan &P, int *count,
+                              const long long *offset, void *val, int *row, int *col, int round6,
+                              hipStream_t st) {
+  const long npix = long(P.nx) * P.ny;
+  dim3 grid(unsigned((npix + kThreads - 1) / kThreads), unsigned(P.nviews));
+  T *v = static_cast<T *>(val);
+#define TRTT_SPARSE_CASE(K, TB, Q)                                                              \
+  if (kind == K && (K == LF2D || (table == TB && quirk == Q))) {                               \
+    hipLaunchKernelGGL((sparse_columns<T, K, TB, Q, FILL>), grid, dim3(kThreads), 0, st, P,     \
+                       count, offset, v, row, col, round6);                                     \
+    return hipGetLastError();                                                                   \
+  }
+  TRTT_SPARSE_CASE(FB2D, false, true)
+  TRTT_SPARSE_CASE(FB2D, true, false)
+  TRTT_SPARSE_CASE(PB2D, false, true)
+  TRTT_SPARSE_CASE(PB2D, true, false)
+  TRTT_SPARSE_CASE(LF2D, false, false)
+#undef TRTT_SPARSE_CASE
+  return hipErrorInvalidValue;
+}
+
+// fill = false: count[view * npix + pixel] = entries; fill = true: write them
+// at offset[view * npix + pixel].  Grid (pixels / 256, views).
+hipError_t launch_sparse_columns(int kind, bool table, bool quirk, bool f64, const DevPlan &P,
+                                 bool fill, int *count, const long long *offset, void *val,
+                                 int *row, int *col, int round6, hipStream_t st) {
+  if (f64)
+    return fill ? launch_sparse_fill<double, true>(kind, table, quirk, P, count, offset, val, row,
+                                                   col, round6, st)
+                : launch_sparse_fill<double, false>(kind, table, quirk, P, count, offset, val,
+                                                    row, col, round6, st);
+  return fill ? launch_sparse_fill<float, true>(kind, table, quirk, P, count, offset, val, row, col,
+                                                round6, st)
+              : launch_sparse_fill<float, false>(kind, table, quirk, P, count, offset, val, row,
+                                                 col, round6, st);
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers (explicit instantiation table)
 // ---------------------------------------------------------------------------
 template <typename T, int KIND, bool TABLE, bool QUIRK>

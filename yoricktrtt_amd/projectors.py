@@ -315,6 +315,12 @@ class Projector:
     def error_flag(self) -> int:
         return self.lib.trtt_hip_plan_error_flag(self.handle)
 
+    def sparse_matrix(self, decimals: int = 6) -> "SparseMatrix":
+        """System matrix of this 2-D plan on the device (_trtt_2D_sparse_matrix,
+        Ytrtt_2D_projectors.i:295-369); ``decimals=6`` keeps the reference's
+        6-decimal coefficients, ``-1`` exact ones."""
+        return SparseMatrix(self, decimals)
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.trtt_hip_plan_destroy(self.handle)
@@ -355,3 +361,90 @@ class LFProjector(Projector):
     @property
     def sino_shape(self):
         return (self.nviews, self.params.detectorDimension)
+
+
+class SparseMatrix:
+    """Device-resident sparse system matrix of a 2-D plan.
+
+    Rows are (view, bin) = view * nv + bin, columns pixels j * nx + i.
+    ``mvmult(img)`` returns ``M img`` as a sinogram and ``mvmult(sino, 1)``
+    returns ``M' sino`` as an image, like Yeti's ``mvmult(M, x[, 1])`` on the
+    matrix of Ytrtt_2D_projectors.i:362-366; ``forward`` / ``backward``
+    accumulate into given tensors.  ``coo()`` returns the entries in the
+    reference's order with 1-based indices (MCops_coefs, MCops_row_indices,
+    MCops_col_indices).
+    """
+
+    def __init__(self, proj: Projector, decimals: int = 6):
+        self.proj = proj
+        self.lib = proj.lib
+        self.torch = proj.torch
+        self.handle = self.lib.trtt_hip_sparse_create(proj.handle, int(decimals))
+        if not self.handle:
+            raise RuntimeError("sparse matrix creation failed: " + _lib.last_error())
+
+    @property
+    def nnz(self) -> int:
+        return int(self.lib.trtt_hip_sparse_nnz(self.handle))
+
+    # operator interface of the plan (the data term runs on either)
+    @property
+    def vol_shape(self):
+        return self.proj.vol_shape
+
+    @property
+    def sino_shape(self):
+        return self.proj.sino_shape
+
+    @property
+    def dtype(self):
+        return self.proj.dtype
+
+    def weighted_residual(self, sino, y, w, fx, stream=None):
+        self.proj.weighted_residual(sino, y, w, fx, stream)
+
+    def coo(self):
+        n = self.nnz
+        coefs = np.empty(n, dtype=np.float64)
+        rows = np.empty(n, dtype=np.int64)
+        cols = np.empty(n, dtype=np.int64)
+        if self.lib.trtt_hip_sparse_coo(self.handle, _ptr(coefs), rows.ctypes.data_as(_lib.lp),
+                                        cols.ctypes.data_as(_lib.lp)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return coefs, rows, cols
+
+    def forward(self, img, sino=None, stream=None):
+        P = self.proj
+        if sino is None:
+            sino = self.torch.zeros(P.sino_shape, dtype=P.dtype, device=img.device)
+        P._check(img, P.vol_shape)
+        P._check(sino, P.sino_shape)
+        if self.lib.trtt_hip_sparse_forward(self.handle, C.c_void_p(img.data_ptr()),
+                                            C.c_void_p(sino.data_ptr()), P._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return sino
+
+    def backward(self, sino, img=None, stream=None):
+        P = self.proj
+        if img is None:
+            img = self.torch.zeros(P.vol_shape, dtype=P.dtype, device=sino.device)
+        P._check(img, P.vol_shape)
+        P._check(sino, P.sino_shape)
+        if self.lib.trtt_hip_sparse_backward(self.handle, C.c_void_p(sino.data_ptr()),
+                                             C.c_void_p(img.data_ptr()), P._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return img
+
+    def mvmult(self, x, transpose: int = 0):
+        return self.backward(x) if transpose else self.forward(x)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.trtt_hip_sparse_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
