@@ -239,6 +239,24 @@ int trtt_hip_sparse_forward(trtt_hip_sparse_t *m, const void *img, void *sino, v
 int trtt_hip_sparse_backward(trtt_hip_sparse_t *m, const void *sino, void *img, void *stream);
 int trtt_hip_sparse_destroy(trtt_hip_sparse_t *m);
 
+/* Temporal interpolation of a series of nt frames (images or volumes of npix
+ * values, plan precision): the B_k operators of the dynamic systems,
+ * _trtt_2D_time_interpolator (Ytrtt_2D_projectors.i:761-814) and the
+ * interpolator of Ytrtt_3D_projectors.i:254-375, whose taps come from
+ * spl_interp_coefs_new_spline.  View k has `taps` taps (idx[k*taps+q] in
+ * [0, nt), weight w[k*taps+q]; host arrays, copied):
+ *   job 0: out[k] += sum_q w[k*taps+q] in[idx[k*taps+q]]   (nviews output frames)
+ *   job != 0: the transpose, out[t] += sum over the taps naming t of w in[k]
+ *             (nt output frames; repeated phases of one view add up).
+ * Frames <= 65535, phases <= 65535, 1..64 taps.  apply is asynchronous on
+ * `stream` (device pointers, in != out). */
+typedef struct trtt_hip_tinterp trtt_hip_tinterp_t;
+trtt_hip_tinterp_t *trtt_hip_tinterp_create(int precision, long npix, long nt, long nviews,
+                                            int taps, const int *idx, const double *w);
+int trtt_hip_tinterp_apply(trtt_hip_tinterp_t *h, int job, const void *in, void *out,
+                           void *stream);
+int trtt_hip_tinterp_destroy(trtt_hip_tinterp_t *h);
+
 long trtt_hip_plan_nviews(const trtt_hip_plan_t *plan);
 long trtt_hip_plan_volume_size(const trtt_hip_plan_t *plan);   /* nt*nz*ny*nx */
 long trtt_hip_plan_sino_size(const trtt_hip_plan_t *plan);     /* nviews*nv*nu */

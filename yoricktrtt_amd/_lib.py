@@ -77,6 +77,9 @@ _SIGS = {
     "trtt_hip_sparse_forward": ([vp, vp, vp, vp], I),
     "trtt_hip_sparse_backward": ([vp, vp, vp, vp], I),
     "trtt_hip_sparse_destroy": ([vp], I),
+    "trtt_hip_tinterp_create": ([I, L, L, L, I, ip, dp], vp),
+    "trtt_hip_tinterp_apply": ([vp, I, vp, vp, vp], I),
+    "trtt_hip_tinterp_destroy": ([vp], I),
     "trtt_hip_tv": ([vp, vp, vp, vp, L, L, L, L, D, D, D, I, vp], I),
     "trtt_hip_bspline_filter": ([vp, vp, L, L, L, L, I, I, vp], I),
     "trtt_unet_conv_fwd": ([vp, vp, vp, vp, vp, I, I, I, I, I, I, I, I, I, vp], I),

@@ -39,6 +39,9 @@ hipError_t sparse_compress(bool f64, const unsigned *key_sorted, const unsigned 
                            int *ptr, long n, int n_major, hipStream_t st);
 hipError_t sparse_spmv(bool f64, const int *ptr, const int *idx, const void *val, const void *x,
                        void *y, int n_major, hipStream_t st);
+hipError_t launch_tinterp(bool f64, int job, const void *in, void *out, long npix, int nframes,
+                          int taps, const int *idx, const void *w, const int *ptr,
+                          const int *src, const void *wb, hipStream_t st);
 }  // namespace trtt
 
 using trtt::ViewConst;
@@ -1058,6 +1061,121 @@ int trtt_hip_sparse_backward(trtt_hip_sparse_t *m, const void *sino, void *img, 
 
 int trtt_hip_sparse_destroy(trtt_hip_sparse_t *m) {
   sparse_release(m);
+  return TRTT_SUCCESS;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Part 2c -- temporal interpolation of frame series
+// ===========================================================================
+struct trtt_hip_tinterp {
+  int device = 0;
+  bool f64 = false;
+  long npix = 0;
+  int nt = 0, nviews = 0, taps = 0;
+  int *idx = nullptr;   // [nviews][taps] phase of each tap
+  void *w = nullptr;    // [nviews][taps] weights (plan precision)
+  int *ptr = nullptr;   // [nt + 1] per-phase list of (view, weight)
+  int *src = nullptr;
+  void *wb = nullptr;
+};
+
+namespace {
+
+void tinterp_release(trtt_hip_tinterp *h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  for (void *q : {static_cast<void *>(h->idx), h->w, static_cast<void *>(h->ptr),
+                  static_cast<void *>(h->src), h->wb})
+    if (q) hipFree(q);
+  delete h;
+}
+
+// Weights in the plan precision, uploaded.
+bool upload_weights(const std::vector<double> &w, bool f64, void *&dst) {
+  const size_t n = std::max<size_t>(w.size(), 1);
+  if (!hip_ok(hipMalloc(&dst, n * (f64 ? 8 : 4)), "hipMalloc")) return false;
+  if (w.empty()) return true;
+  if (f64) return hip_ok(hipMemcpy(dst, w.data(), 8 * w.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  std::vector<float> f(w.begin(), w.end());
+  return hip_ok(hipMemcpy(dst, f.data(), 4 * f.size(), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+template <typename U>
+bool upload_vec(const std::vector<U> &v, U *&dst) {
+  if (!dev_alloc(dst, std::max<size_t>(v.size(), 1))) return false;
+  return v.empty() ||
+         hip_ok(hipMemcpy(dst, v.data(), sizeof(U) * v.size(), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+}  // namespace
+
+extern "C" {
+
+trtt_hip_tinterp_t *trtt_hip_tinterp_create(int precision, long npix, long nt, long nviews,
+                                            int taps, const int *idx, const double *w) {
+  if (!idx || !w) {
+    fail("null argument");
+    return nullptr;
+  }
+  if ((precision != TRTT_HIP_F32 && precision != TRTT_HIP_F64) || npix <= 0 || nt <= 0 ||
+      nt > 65535 || nviews <= 0 || nviews > 65535 || taps <= 0 || taps > 64 ||
+      nviews * taps >= long(INT_MAX)) {
+    fail("invalid temporal interpolation sizes (frames and phases <= 65535, 1..64 taps)");
+    return nullptr;
+  }
+  const long n = nviews * taps;
+  for (long e = 0; e < n; ++e)
+    if (idx[e] < 0 || idx[e] >= nt) {
+      fail("temporal interpolation phase index out of range");
+      return nullptr;
+    }
+  // per-phase lists in (view, tap) order
+  std::vector<int> ptr(nt + 1, 0), src(n);
+  std::vector<double> wb(n);
+  for (long e = 0; e < n; ++e) ++ptr[idx[e] + 1];
+  for (long t = 0; t < nt; ++t) ptr[t + 1] += ptr[t];
+  std::vector<int> fillp(ptr.begin(), ptr.end() - 1);
+  for (long e = 0; e < n; ++e) {
+    const int slot = fillp[idx[e]]++;
+    src[slot] = int(e / taps);
+    wb[slot] = w[e];
+  }
+  if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return nullptr;
+  trtt_hip_tinterp *h = new trtt_hip_tinterp;
+  h->device = g_device;
+  h->f64 = precision == TRTT_HIP_F64;
+  h->npix = npix;
+  h->nt = int(nt);
+  h->nviews = int(nviews);
+  h->taps = taps;
+  const std::vector<int> iv(idx, idx + n);
+  const std::vector<double> wv(w, w + n);
+  if (!upload_vec(iv, h->idx) || !upload_weights(wv, h->f64, h->w) ||
+      !upload_vec(ptr, h->ptr) || !upload_vec(src, h->src) ||
+      !upload_weights(wb, h->f64, h->wb)) {
+    tinterp_release(h);
+    return nullptr;
+  }
+  return h;
+}
+
+int trtt_hip_tinterp_apply(trtt_hip_tinterp_t *h, int job, const void *in, void *out,
+                           void *stream) {
+  if (!h || !in || !out) return fail("null argument");
+  if (in == out) return fail("in and out must differ");
+  const int frames = job == 0 ? h->nviews : h->nt;
+  return hip_ok(trtt::launch_tinterp(h->f64, job == 0 ? 0 : 1, in, out, h->npix, frames, h->taps,
+                                     h->idx, h->w, h->ptr, h->src, h->wb,
+                                     static_cast<hipStream_t>(stream)),
+                "temporal interpolation")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_tinterp_destroy(trtt_hip_tinterp_t *h) {
+  tinterp_release(h);
   return TRTT_SUCCESS;
 }
 

@@ -139,6 +139,74 @@ unsigned grid_for(long n) {
 }  // namespace regul
 }  // namespace trtt
 
+namespace trtt {
+namespace regul {
+
+// Temporal interpolation of a series of frames (images or volumes of npix
+// values): B_k of the dynamic systems (_trtt_2D_time_interpolator,
+// Ytrtt_2D_projectors.i:761-814; Ytrtt_3D_projectors.i:254-375), with taps
+// (phase index, weight) per view.  Forward: frame k = sum over its taps of
+// w x[phase]; transpose: phase t += sum over the taps that name t of w y[k],
+// gathered from a per-phase list (no atomics, fixed order).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tinterp_fwd(const T *__restrict__ x, T *__restrict__ out,
+                                                        long npix, int taps,
+                                                        const int *__restrict__ idx,
+                                                        const T *__restrict__ w) {
+  const long p = long(blockIdx.x) * kThreads + threadIdx.x;
+  const int k = blockIdx.y;
+  if (p >= npix) return;
+  T s = T(0);
+  for (int q = 0; q < taps; ++q) s += w[k * taps + q] * x[long(idx[k * taps + q]) * npix + p];
+  out[long(k) * npix + p] += s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tinterp_bwd(const T *__restrict__ y, T *__restrict__ out,
+                                                        long npix, const int *__restrict__ ptr,
+                                                        const int *__restrict__ src,
+                                                        const T *__restrict__ w) {
+  const long p = long(blockIdx.x) * kThreads + threadIdx.x;
+  const int t = blockIdx.y;
+  if (p >= npix) return;
+  T s = T(0);
+  for (int e = ptr[t]; e < ptr[t + 1]; ++e) s += w[e] * y[long(src[e]) * npix + p];
+  out[long(t) * npix + p] += s;
+}
+
+}  // namespace regul
+
+// job 0: out[k] += sum_q w[k taps + q] in[idx[k taps + q]] for the nframes
+// views; job 1: out[t] += sum over the list of phase t of w in[src].
+hipError_t launch_tinterp(bool f64, int job, const void *in, void *out, long npix, int nframes,
+                          int taps, const int *idx, const void *w, const int *ptr,
+                          const int *src, const void *wb, hipStream_t st) {
+  using namespace regul;
+  dim3 grid(unsigned((npix + kThreads - 1) / kThreads), unsigned(nframes));
+  if (job == 0) {
+    if (f64)
+      hipLaunchKernelGGL(tinterp_fwd<double>, grid, dim3(kThreads), 0, st,
+                         static_cast<const double *>(in), static_cast<double *>(out), npix, taps,
+                         idx, static_cast<const double *>(w));
+    else
+      hipLaunchKernelGGL(tinterp_fwd<float>, grid, dim3(kThreads), 0, st,
+                         static_cast<const float *>(in), static_cast<float *>(out), npix, taps,
+                         idx, static_cast<const float *>(w));
+  } else {
+    if (f64)
+      hipLaunchKernelGGL(tinterp_bwd<double>, grid, dim3(kThreads), 0, st,
+                         static_cast<const double *>(in), static_cast<double *>(out), npix, ptr,
+                         src, static_cast<const double *>(wb));
+    else
+      hipLaunchKernelGGL(tinterp_bwd<float>, grid, dim3(kThreads), 0, st,
+                         static_cast<const float *>(in), static_cast<float *>(out), npix, ptr, src,
+                         static_cast<const float *>(wb));
+  }
+  return hipGetLastError();
+}
+
+}  // namespace trtt
+
 extern "C" {
 
 int trtt_hip_tv(const void *x, void *g, void *work, double *f_dev, long nx, long ny, long nz,

@@ -448,3 +448,155 @@ class SparseMatrix:
             self.close()
         except Exception:
             pass
+
+
+def temporal_weights(t_index, nt: int, t_deg: int = 1):
+    """Taps of the B_k operators: periodic B-spline interpolation in time.
+
+    View k at time ``t_index[k]`` (phase units) reads phases
+    j = j0, ..., j0 + t_deg (modulo nt), j0 = floor(t - (t_deg + 1)/2) + 1,
+    with weights beta^t_deg(t - j).  Degree 1 is the linear interpolation of
+    Ytrtt_3D_projectors.i:301-311 (phases floor(t) and floor(t) + 1 modulo
+    nt).  Degrees 2 and 3 take the place of SPL's
+    spl_interp_coefs_new_spline (Ytrtt_2D_projectors.i:747), whose source is
+    not available: periodic boundaries here.  Returns idx (int32) and w
+    (float64), both [nviews][t_deg + 1]."""
+    from .geometry import bspline
+
+    t = np.asarray(t_index, dtype=np.float64).reshape(-1)
+    if t_deg < 1 or t_deg > 3:
+        raise ValueError("t_deg must be 1, 2 or 3")
+    if np.any(t < 0) or np.any(t > nt):
+        raise ValueError("time index out of [0, nt]")
+    j0 = np.floor(t - 0.5 * (t_deg + 1)).astype(np.int64) + 1
+    j = j0[:, None] + np.arange(t_deg + 1)[None, :]
+    w = bspline(t[:, None] - j, t_deg)
+    return np.ascontiguousarray(j % nt, dtype=np.int32), np.ascontiguousarray(w)
+
+
+class TimeInterpolator:
+    """B_k of every view on the device (trtt_hip_tinterp_*): ``forward(x)``
+    maps a series [nt, ...] to one frame per view [nviews, ...];
+    ``backward(frames)`` is the transpose (_trtt_2D_time_interpolator,
+    Ytrtt_2D_projectors.i:761-814)."""
+
+    def __init__(self, frame_shape, nt: int, idx, w, precision: str = "f32"):
+        import torch
+
+        self.torch = torch
+        self.lib = _lib.load()
+        self.precision = _lib.F64 if precision == "f64" else _lib.F32
+        self.dtype = torch.float64 if self.precision == _lib.F64 else torch.float32
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        w = _f64(w)
+        self.frame_shape = tuple(frame_shape)
+        self.nt, (self.nviews, self.taps) = int(nt), idx.shape
+        npix = int(np.prod(self.frame_shape))
+        self.handle = self.lib.trtt_hip_tinterp_create(self.precision, npix, self.nt, self.nviews,
+                                                       self.taps, idx.ctypes.data_as(_lib.ip),
+                                                       _ptr(w))
+        if not self.handle:
+            raise RuntimeError("temporal interpolator creation failed: " + _lib.last_error())
+
+    def _apply(self, job, src, dst, stream):
+        s = stream if stream is not None else self.torch.cuda.current_stream()
+        if self.lib.trtt_hip_tinterp_apply(self.handle, job, C.c_void_p(src.data_ptr()),
+                                           C.c_void_p(dst.data_ptr()),
+                                           C.c_void_p(s.cuda_stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return dst
+
+    def forward(self, x, frames=None, stream=None):
+        if frames is None:
+            frames = self.torch.zeros((self.nviews,) + self.frame_shape, dtype=self.dtype,
+                                      device=x.device)
+        return self._apply(0, x, frames, stream)
+
+    def backward(self, frames, x=None, stream=None):
+        if x is None:
+            x = self.torch.zeros((self.nt,) + self.frame_shape, dtype=self.dtype,
+                                 device=frames.device)
+        return self._apply(1, frames, x, stream)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.trtt_hip_tinterp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DynamicProjector:
+    """C_k = A_k B_k over every view of a geometry for a series of nt
+    frames (Ytrtt_2D_projectors.i:695-814, Tomography_2D_dyn): the frames of
+    the views are interpolated by a :class:`TimeInterpolator` and projected
+    by a plan with one frame per view; the transpose runs back the same way.
+    Linear interpolation (``t_deg`` 1) needs no frame buffer: the plan's own
+    two-phase weights carry it."""
+
+    def __init__(self, geom: Geometry, nt: int, t_index, t_deg: int = 1, model: str = "cubic",
+                 precision: str = "f32"):
+        import torch
+
+        self.torch = torch
+        self.geom = geom
+        self.nt = int(nt)
+        idx, w = temporal_weights(t_index, nt, t_deg)
+        self.linear = t_deg == 1
+        V = geom.nviews
+        if self.linear:
+            self.P = Projector(geom, model=model, precision=precision, nt=nt, t0=idx[:, 0],
+                               t1=idx[:, 1], w0=w[:, 0], w1=w[:, 1])
+            self.B = None
+        else:
+            one = np.arange(V, dtype=np.int32)
+            self.P = Projector(geom, model=model, precision=precision, nt=V, t0=one, t1=one,
+                               w0=np.ones(V), w1=np.zeros(V))
+            frame = self.P.vol_shape[1:]
+            self.B = TimeInterpolator(frame, nt, idx, w, precision)
+            self.frames = torch.zeros((V,) + tuple(frame), dtype=self.P.dtype, device="cuda")
+        self.dtype = self.P.dtype
+
+    @property
+    def vol_shape(self):
+        g = self.geom
+        return (self.nt,) + ((g.nz, g.ny, g.nx) if g.is3d else (g.ny, g.nx))
+
+    @property
+    def sino_shape(self):
+        return self.P.sino_shape
+
+    def _zero_frames(self, stream):
+        if stream is None:
+            self.frames.zero_()
+        else:
+            with self.torch.cuda.stream(stream):
+                self.frames.zero_()
+
+    def forward(self, x, sino=None, stream=None):
+        if self.linear:
+            return self.P.forward(x, sino, stream)
+        self._zero_frames(stream)
+        self.B.forward(x, self.frames, stream)
+        return self.P.forward(self.frames, sino, stream)
+
+    def backward(self, sino, x=None, stream=None):
+        if self.linear:
+            return self.P.backward(sino, x, stream)
+        if x is None:
+            x = self.torch.zeros(self.vol_shape, dtype=self.dtype, device=sino.device)
+        self._zero_frames(stream)
+        self.P.backward(sino, self.frames, stream)
+        return self.B.backward(self.frames, x, stream)
+
+    def weighted_residual(self, sino, y, w, fx, stream=None):
+        self.P.weighted_residual(sino, y, w, fx, stream)
+
+    def close(self):
+        self.P.close()
+        if self.B is not None:
+            self.B.close()
