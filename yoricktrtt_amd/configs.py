@@ -33,6 +33,11 @@ and results.  Configuration 5 (the learned warm start) lives in
    ellipsoid moves by 10 sin(2 pi p / 10) mm along z; all phases in one
    plan (one launch sequence per projection, the data term captured in one
    HIP graph per evaluation), TV along t as well; 30 iterations.
+6. 2-D+t fan beam (the dynamic scripts of Tomography_2D_dyn): 16 phases of
+   256 x 256 pixels of 1 mm, a moving inner ellipse; 720 views over 360 deg
+   acquired during 127 s of a 7 s motion; 512 bins of 1 mm, Rsc = 1000 mm,
+   Rsd = 1536 mm; cubic B-spline in time; data = this operator's projection
+   with Poisson noise; TV along t as well; 30 iterations.
 """
 from __future__ import annotations
 
@@ -374,11 +379,69 @@ def config4(iters=30, tv_weight=5.0, tv_eps=0.05, wt=1.0, photons=1e5, mu_scale=
                 fwd_ms=1e3 * fwd, bwd_ms=1e3 * bwd, pairs_per_s=1.0 / (fwd + bwd), **sol)
 
 
-RUNNERS = {1: config1, 2: config2, 3: config3, 4: config4}
+def dynamic_series_2d(n, nt, seed=6):
+    """nt frames of an n x n image: 12 fixed ellipses (seed) and an inner
+    ellipse of value 0.8 whose centre moves by 0.08 n sin(2 pi p / nt)
+    along y (a breathing motion)."""
+    from .phantoms import ellipses_2d
+
+    base, _ = ellipses_2d(n, 12, seed, centre_frac=0.5, axes_frac=(0.05, 0.3),
+                          dtype=np.float32)
+    yy, xx = np.meshgrid(np.arange(n) - (n - 1) / 2, np.arange(n) - (n - 1) / 2, indexing="ij")
+    out = np.empty((nt, n, n), dtype=np.float32)
+    for p in range(nt):
+        cy = 0.08 * n * math.sin(2 * math.pi * p / nt)
+        inner = ((xx / (0.1 * n)) ** 2 + ((yy - cy) / (0.07 * n)) ** 2) <= 1.0
+        out[p] = base + np.where(inner, np.float32(0.8), np.float32(0))
+    return out
+
+
+def config6(iters=30, tv_weight=2.0, tv_eps=0.01, wt=1.0, photons=1e5, mu_scale=0.02):
+    """2-D+t fan beam (Tomography_2D_dyn, Ytrtt2D_script_reconstructions_
+    dynamic_new.i:5-59): views over one turn acquired during tacq = 127 s of
+    a motion of period tcycl = 7 s, t_index = (t % tcycl) nt / tcycl
+    (Ytrtt2D_system_dyn.i:500-509); cubic temporal B-spline (t_deg 3)."""
+    from .geometry import fan_geometry_2d
+    from .phantoms import transmission_noise
+    from .projectors import DynamicProjector
+    from .reconstruction import DataTerm, Objective, TotalVariation
+
+    torch = _torch()
+    n, nt, V, nv, Rsc, Rsd, tacq, tcycl = 256, 16, 720, 512, 1000.0, 1536.0, 127.0, 7.0
+    theta = np.arange(V) * (2 * math.pi / V)
+    t = np.arange(V) * (tacq / (V - 1))
+    t_index = (t % tcycl) * (nt / tcycl)
+    geo = fan_geometry_2d(n, n, 1.0, nv, 1.0, theta, Rsc, Rsd)
+    dev = torch.device("cuda")
+    truth = torch.from_numpy(dynamic_series_2d(n, nt)).to(dev)
+    P = DynamicProjector(geo, nt, t_index, t_deg=3, model="cubic", precision="f32")
+    sino = torch.zeros(P.sino_shape, dtype=torch.float32, device=dev)
+    fwd, bwd = _projection_times(P, truth, sino)
+    sino.zero_()
+    P.forward(truth, sino)
+    gen = torch.Generator(device=dev).manual_seed(6)
+    y, w = transmission_noise(sino, photons, mu_scale, generator=gen)
+    D = DataTerm(P, y, w)
+    shape = (nt, n, n)
+    # a series of nt single-slice frames: B-spline sampling and TV in x, y
+    # within each frame, TV along t across frames
+    series = (nt, 1, n, n)
+    tv = TotalVariation(series, tv_weight, tv_eps, wt=wt, samples=True, dtype=torch.float32,
+                        device=dev)
+    D.capture(torch.zeros(shape, device=dev), torch.zeros(shape, device=dev))
+    _, sol = _solve(Objective(D, [tv]), torch.zeros(shape, device=dev), iters, truth,
+                    _sampler(series, torch.float32, dev))
+    P.close()
+    return dict(config=6, workload="2D+t fan 256^2 x 1 mm, 16 phases, 720 views / 360 deg over "
+                "127 s of a 7 s motion, 512 bins 1 mm, Rsc 1000, Rsd 1536, cubic in t, FP32",
+                fwd_ms=1e3 * fwd, bwd_ms=1e3 * bwd, **sol)
+
+
+RUNNERS = {1: config1, 2: config2, 3: config3, 4: config4, 6: config6}
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(description="run benchmark configurations 1-4 on the GPU")
+    ap = argparse.ArgumentParser(description="run benchmark configurations 1-4 and 6 on the GPU")
     ap.add_argument("configs", nargs="*", type=int, default=[1, 2, 3, 4])
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--out", default=None, help="append JSON lines here")
