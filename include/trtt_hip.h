@@ -109,6 +109,20 @@ int trtt_xform3d_invert(trtt_xform3d_t *dst, const trtt_xform3d_t *src);
 int trtt_xform3d_get_coefficients(double dst[12], const trtt_xform3d_t *src);
 int trtt_xform3d_set_coefficients(trtt_xform3d_t *dst, const double src[12]);
 
+/* Distance-driven parallel-beam projector (plugin/trtt_3D_projectors.c:
+ * 1077-1173, prototype trtt_3D_projectors.h): job 0 accumulates the forward
+ * projection, any other job the reference's transpose.  That transpose is
+ * not the adjoint when a footprint spans three or more pixels along v: its
+ * loop over the inner pixels (trtt_3D_projectors.c:909-916) advances the u
+ * counter, and it is reproduced as is. */
+int trtt_DD_PB3D(double *out, double *in, long nx, long ny, long nz, double s_scl, long nu,
+                 long nv, double u_scl, double v_scl, double *xos, double *xsd, int job);
+/* The exact adjoint of trtt_DD_PB3D's forward projection (no reference
+ * counterpart; same arguments without job): out (volume) += A' in. */
+int trtt_hip_DD_PB3D_adjoint(double *out, double *in, long nx, long ny, long nz, double s_scl,
+                             long nu, long nv, double u_scl, double v_scl, double *xos,
+                             double *xsd);
+
 /* Sparse matrix-vector product out[i[k]-1] += a[k]*in[j[k]-1] (1-based indices).
  * Replaces plugin/trtt_common.c:62-69. */
 int trtt_mvmult(double *out, double *in, long number, double *a, long *i, long *j);

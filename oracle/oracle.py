@@ -69,6 +69,8 @@ class Oracle:
         lib.orc_sb3d_count_pairs.argtypes = [L, L, L, D, I, L, L, D, D, dp, dp, I]
         lib.orc_sb3d_count_pairs.restype = D
         lib.orc_lf2d.argtypes = [C.POINTER(LF2DParams), D, D, dp, dp, I, I]
+        lib.orc_dd_pb3d.argtypes = [dp, dp, L, L, L, D, L, L, D, D, dp, dp, I, I]
+        lib.orc_dd_pb3d.restype = I
         lib.orc_lf2d.restype = None
         self.lib = lib
 
@@ -123,6 +125,15 @@ class Oracle:
     def count_pairs(self, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, cone, deg=3) -> float:
         return self.lib.orc_sb3d_count_pairs(nx, ny, nz, s, deg, nu, nv, us, vs, _p(_f64(xos)),
                                              _p(_f64(xsd)), int(cone))
+
+    def dd_pb3d(self, out, inp, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, job, adjoint=False):
+        """One view of trtt_DD_PB3D; accumulates into ``out``.  job != 0 is
+        the reference's transpose, or the exact adjoint with ``adjoint``."""
+        xos, xsd, inp = _f64(xos), _f64(xsd), _f64(inp)
+        assert out.dtype == np.float64 and out.flags.c_contiguous
+        self.lib.orc_dd_pb3d(_p(out), _p(inp), nx, ny, nz, s, nu, nv, us, vs, _p(xos), _p(xsd),
+                             int(job), int(adjoint))
+        return out
 
     def lf2d(self, params: LF2DParams, beta, offset, src, dst, clr, job):
         src = _f64(src)

@@ -611,6 +611,87 @@ void orc_lf2d(const orc_lf2d_t *p, double beta, double det_offset,
 /* ------------------------------------------------------------------------ */
 
 /* Same projector with the footprint given field by field (for ctypes). */
+/* ------------------------------------------------------------------------ */
+/* Distance-driven parallel-beam projector                                    */
+/* (trtt_DD_PB3D, trtt_3D_projectors.c:1077-1173; footprint burns :406-525,   */
+/* transpose :817-940).  Each voxel projects to the rectangle                 */
+/* [uc - Du, uc + Du] x [vc - Dv, vc + Dv] with its corners clamped to        */
+/* [0, n-1]; the weight of a pixel is the length of its overlap with the     */
+/* rectangle along each axis, times u_scl / v_scl and h_norm.                 */
+/* ------------------------------------------------------------------------ */
+
+/* Overlap of pixel `left + p` with [b1, b2] (clamped bounds), scaled.
+ * `stuck` restates the reference transpose along v (:909-916), whose loop
+ * over the pixels inside the rectangle advances the u counter instead of the
+ * v one: every inner pixel writes the same slot, the last slot write wins and
+ * the slots after it stay zero. */
+static double dd_overlap(double b1, double b2, long p, double scl, int stuck)
+{
+    const double up = ceil(b1), down = floor(b2);
+    const long inner = (long)(down - up); /* pixels fully inside, or < 0 */
+    const int low = up > b1, high = down < b2;
+    if (inner < 0) return p == 0 ? scl * (b2 - b1) : 0.0;
+    if (!stuck) {
+        const long n = (long)(ceil(b2) - floor(b1));
+        if (low && p == 0) return scl * (up - b1);
+        if (high && p == n - 1) return scl * (b2 - down);
+        return scl;
+    }
+    /* slots: [0] the lower partial pixel if any, then one shared slot */
+    if (low && p == 0) return scl * (up - b1);
+    if (p == (low ? 1 : 0)) return high ? scl * (b2 - down) : (inner > 0 ? scl : 0.0);
+    return 0.0;
+}
+
+int orc_dd_pb3d(double *out, const double *in, long nx, long ny, long nz, double s,
+                long nu, long nv, double us, double vs, const double *xos,
+                const double *xsd, int job, int adjoint)
+{
+#pragma STDC FP_CONTRACT OFF
+    const double a1inv = -1.0 * (1.0 / xsd[0]);
+    const double b1 = xsd[5] + a1inv * xsd[4] * xsd[1], c1 = xsd[9] + a1inv * xsd[8] * xsd[1];
+    const double b2 = xsd[6] + a1inv * xsd[4] * xsd[2], c2 = xsd[10] + a1inv * xsd[8] * xsd[2];
+    const double b3 = xsd[7] + a1inv * xsd[4] * xsd[3], c3 = xsd[11] + a1inv * xsd[8] * xsd[3];
+    const double v_x = b1 * xos[4] + b2 * xos[8], u_x = c1 * xos[4] + c2 * xos[8];
+    const double v_y = b1 * xos[5] + b2 * xos[9], u_y = c1 * xos[5] + c2 * xos[9];
+    const double v_z = b1 * xos[6] + b2 * xos[10], u_z = c1 * xos[6] + c2 * xos[10];
+    const double v_c = b1 * xos[7] + b2 * xos[11] + b3;
+    const double u_c = c1 * xos[7] + c2 * xos[11] + c3;
+    double Du = fabs(u_z);
+    double Dv = fabs(v_y) > fabs(v_x) ? fabs(v_y) : fabs(v_x);
+    const double h = (s * s * s) / (us * vs * Du * Dv);
+    Du = 0.5 * Du;
+    Dv = 0.5 * Dv;
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                const double vc = ((v_z * k + v_c) + v_y * j) + v_x * i;
+                const double uc = ((u_z * k + u_c) + u_y * j) + u_x * i;
+                const double ub1 = fmin(fmax(uc - Du, 0.0), (double)(nu - 1));
+                const double ub2 = fmin(fmax(uc + Du, 0.0), (double)(nu - 1));
+                const double vb1 = fmin(fmax(vc - Dv, 0.0), (double)(nv - 1));
+                const double vb2 = fmin(fmax(vc + Dv, 0.0), (double)(nv - 1));
+                const long nu_e = (long)(ceil(ub2) - floor(ub1));
+                const long nv_e = (long)(ceil(vb2) - floor(vb1));
+                if (nu_e <= 0 || nv_e <= 0) continue;
+                const long corner = (long)floor(vb1) * nu + (long)floor(ub1);
+                const long vox = (k * ny + j) * nx + i;
+                const int stuck = job != 0 && !adjoint;
+                for (long m = 0; m < nv_e; ++m) {
+                    const double cv = dd_overlap(vb1, vb2, m, vs, stuck);
+                    for (long l = 0; l < nu_e; ++l) {
+                        const double cu = dd_overlap(ub1, ub2, l, us, 0);
+                        const long pix = corner + m * nu + l;
+                        if (job == 0)
+                            out[pix] += h * cv * cu * in[vox];
+                        else
+                            out[vox] += h * cv * cu * in[pix];
+                    }
+                }
+            }
+    return 0;
+}
+
 int orc_sb3d_ex(double *out, const double *in, long nx, long ny, long nz,
                 double s, int deg, int kind, const double *cum, long m,
                 double step, double coord0, long nu, long nv, double us,

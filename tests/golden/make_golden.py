@@ -10,6 +10,8 @@ and records inputs and outputs of every projector on small seeded cases:
 * ``proj2d.npz``    trtt_{PB,FB}2D[_cubic], job 0 and 1, three geometries
 * ``sparse2d.npz``  trtt_{PB,FB}2D_sparser: the coefficient column of every
                     pixel on the views of the 2-D geometries
+* ``dd3d.npz``      trtt_DD_PB3D, job 0 and 1, parallel geometries with pixels
+                    smaller and larger than the voxels
 
 The per-view XOS/XSD pairs are composed with the reference transform code in
 the same sequence as the Yorick system builders
@@ -95,6 +97,8 @@ def load():
     cub3 = [dp, dp, L, L, L, d, I, L, L, d, d, dp, dp, I]
     tab2 = [dp, dp, L, L, d, I, dp, dp, L, d, dp, L, d, dp, dp, I]
     cub2 = [dp, dp, L, L, d, I, L, d, dp, dp, I]
+    lib.trtt_DD_PB3D.argtypes = [dp, dp, L, L, L, d, L, L, d, d, dp, dp, I]
+    lib.trtt_DD_PB3D.restype = I
     sparser = [dp, L, L, L, L, d, I, dp, dp, L, d, dp, L, d, dp, dp]
     for n in ("trtt_PB2D_sparser", "trtt_FB2D_sparser"):
         getattr(lib, n).argtypes = sparser
@@ -352,11 +356,50 @@ def sparse2d_cases(lib, proj2d):
     return out
 
 
+def dd3d_cases(lib):
+    """trtt_DD_PB3D on parallel geometries: pixels smaller than the voxels
+    (footprints over several pixels, which exercises the transpose's v
+    loop), larger ones, and a small detector that clamps the rectangles."""
+    nx, ny, nz = 11, 9, 8
+    s = 1.0
+    rng = np.random.default_rng(8)
+    vol = rng.uniform(0.0, 1.0, size=(nz, ny, nx))
+    cases = {
+        "fine": dict(nu=30, nv=34, us=0.45, vs=0.4, theta=np.array([0.0, 0.35, 1.2, 2.6])),
+        "coarse": dict(nu=9, nv=12, us=1.6, vs=1.3, theta=np.array([0.5, 1.9])),
+        "edge": dict(nu=10, nv=11, us=0.7, vs=0.6, theta=np.array([0.2, 2.3, 4.0])),
+    }
+    out = {"vol": vol, "s": np.array(s)}
+    for name, c in cases.items():
+        nview = len(c["theta"])
+        offs = tuple(np.zeros(nview) for _ in range(5))
+        xos, xsd = ref_geometry_3d(lib, nx, ny, nz, s, c["nu"], c["nv"], c["us"], c["vs"],
+                                   c["theta"], 0.0, 0.0, offs)
+        sino_in = rng.uniform(0.0, 1.0, size=(nview, c["nv"], c["nu"]))
+        fwd = np.zeros((nview, c["nv"], c["nu"]))
+        bwd = np.zeros((nview, nz, ny, nx))
+        for v in range(nview):
+            xo, xs = np.ascontiguousarray(xos[v]), np.ascontiguousarray(xsd[v])
+            assert lib.trtt_DD_PB3D(_p(fwd[v]), _p(vol), nx, ny, nz, s, c["nu"], c["nv"], c["us"],
+                                    c["vs"], _p(xo), _p(xs), 0) == 0
+            si = np.ascontiguousarray(sino_in[v])
+            assert lib.trtt_DD_PB3D(_p(bwd[v]), _p(si), nx, ny, nz, s, c["nu"], c["nv"], c["us"],
+                                    c["vs"], _p(xo), _p(xs), 1) == 0
+        out[f"{name}/xos"] = xos
+        out[f"{name}/xsd"] = xsd
+        out[f"{name}/dims"] = np.array([c["nu"], c["nv"]])
+        out[f"{name}/scl"] = np.array([c["us"], c["vs"]])
+        out[f"{name}/sino_in"] = sino_in
+        out[f"{name}/fwd"] = fwd
+        out[f"{name}/bwd"] = bwd
+    return out
+
+
 def main(names=None):
     if not os.path.exists(LIB):
         sys.exit(f"{LIB} missing: run `make -C oracle` where the reference sources exist")
     lib = load()
-    names = names or ["xform", "footprint", "proj3d", "proj2d", "sparse2d"]
+    names = names or ["xform", "footprint", "proj3d", "proj2d", "sparse2d", "dd3d"]
     if "xform" in names:
         np.savez_compressed(os.path.join(HERE, "xform.npz"), **xform_cases(lib))
     if "footprint" in names:
@@ -372,6 +415,8 @@ def main(names=None):
     if "sparse2d" in names:
         proj2d = np.load(os.path.join(HERE, "proj2d.npz"))
         np.savez_compressed(os.path.join(HERE, "sparse2d.npz"), **sparse2d_cases(lib, proj2d))
+    if "dd3d" in names:
+        np.savez_compressed(os.path.join(HERE, "dd3d.npz"), **dd3d_cases(lib))
     for f in names:
         print(f, os.path.getsize(os.path.join(HERE, f + ".npz")), "bytes")
 

@@ -113,6 +113,40 @@ def test_dropin_2d_vs_reference(lib, golden2d, precision, name, mode):
                 assert out[-1] == 0.0  # last bin quirk
 
 
+@pytest.mark.parametrize("name", ["fine", "coarse", "edge"])
+def test_dropin_dd_pb3d_vs_reference(lib, oracle, precision, name):
+    """trtt_DD_PB3D: forward and the reference's transpose against its golden
+    vectors; trtt_hip_DD_PB3D_adjoint against the oracle's exact adjoint."""
+    import os
+
+    from conftest import GOLDEN
+
+    _say("dd_pb3d", name, precision)
+    g = np.load(os.path.join(GOLDEN, "dd3d.npz"))
+    vol = np.ascontiguousarray(g["vol"])
+    nz, ny, nx = vol.shape
+    nu, nv = (int(v) for v in g[f"{name}/dims"])
+    us, vs = (float(v) for v in g[f"{name}/scl"])
+    for v in range(g[f"{name}/xos"].shape[0]):
+        xos = np.ascontiguousarray(g[f"{name}/xos"][v])
+        xsd = np.ascontiguousarray(g[f"{name}/xsd"][v])
+        y = np.ascontiguousarray(g[f"{name}/sino_in"][v])
+        f = np.zeros((nv, nu))
+        b = np.zeros((nz, ny, nx))
+        a = np.zeros((nz, ny, nx))
+        assert lib.trtt_DD_PB3D(_p(f), _p(vol), nx, ny, nz, 1.0, nu, nv, us, vs, _p(xos), _p(xsd),
+                                0) == 0
+        assert lib.trtt_DD_PB3D(_p(b), _p(y), nx, ny, nz, 1.0, nu, nv, us, vs, _p(xos), _p(xsd),
+                                1) == 0
+        assert lib.trtt_hip_DD_PB3D_adjoint(_p(a), _p(y), nx, ny, nz, 1.0, nu, nv, us, vs, _p(xos),
+                                            _p(xsd)) == 0
+        assert rel_err(f, g[f"{name}/fwd"][v]) < TOL[precision]
+        assert rel_err(b, g[f"{name}/bwd"][v]) < TOL[precision]
+        ra = oracle.dd_pb3d(np.zeros((nz, ny, nx)), y, nx, ny, nz, 1.0, nu, nv, us, vs, xos, xsd, 1,
+                            adjoint=True)
+        assert rel_err(a, ra) < TOL[precision]
+
+
 def test_dropin_accumulates(lib, golden3d):
     """The drop-ins add into `out` like the reference (out += A x)."""
     g = golden3d

@@ -103,6 +103,8 @@ _SIGS = {
     "trtt_xform3d_set_coefficients": ([vp, dp], I),
     "trtt_xform3d_apply": ([dp, vp, dp], I),
     "trtt_mvmult": ([dp, dp, L, dp, lp, lp], I),
+    "trtt_DD_PB3D": ([dp, dp, L, L, L, D, L, L, D, D, dp, dp, I], I),
+    "trtt_hip_DD_PB3D_adjoint": ([dp, dp, L, L, L, D, L, L, D, D, dp, dp], I),
 }
 
 _lib = None

@@ -39,6 +39,8 @@ hipError_t sparse_compress(bool f64, const unsigned *key_sorted, const unsigned 
                            int *ptr, long n, int n_major, hipStream_t st);
 hipError_t sparse_spmv(bool f64, const int *ptr, const int *idx, const void *val, const void *x,
                        void *y, int n_major, hipStream_t st);
+hipError_t launch_dd(bool f64, const DDConst &c, int job, int stuck, const void *in, void *out,
+                     hipStream_t st);
 hipError_t launch_tinterp(bool f64, int job, const void *in, void *out, long npix, int nframes,
                           int taps, const int *idx, const void *w, const int *ptr,
                           const int *src, const void *wb, hipStream_t st);
@@ -1186,6 +1188,110 @@ extern "C" {
 // ===========================================================================
 // Part 1 -- reference plugin symbols
 // ===========================================================================
+// Constants of trtt_DD_PB3D (trtt_3D_projectors.c:1088-1120), in the
+// reference's FP64 expression order.
+static trtt::DDConst dd_consts(long nx, long ny, long nz, double s, long nu, long nv, double us,
+                               double vs, const double *xos, const double *xsd) {
+  trtt::DDConst c;
+  const double a1inv = -1.0 * (1.0 / xsd[0]);
+  const double b1 = xsd[5] + a1inv * xsd[4] * xsd[1], c1 = xsd[9] + a1inv * xsd[8] * xsd[1];
+  const double b2 = xsd[6] + a1inv * xsd[4] * xsd[2], c2 = xsd[10] + a1inv * xsd[8] * xsd[2];
+  const double b3 = xsd[7] + a1inv * xsd[4] * xsd[3], c3 = xsd[11] + a1inv * xsd[8] * xsd[3];
+  c.v_x = b1 * xos[4] + b2 * xos[8];
+  c.u_x = c1 * xos[4] + c2 * xos[8];
+  c.v_y = b1 * xos[5] + b2 * xos[9];
+  c.u_y = c1 * xos[5] + c2 * xos[9];
+  c.v_z = b1 * xos[6] + b2 * xos[10];
+  c.u_z = c1 * xos[6] + c2 * xos[10];
+  c.v_c = b1 * xos[7] + b2 * xos[11] + b3;
+  c.u_c = c1 * xos[7] + c2 * xos[11] + c3;
+  const double Du = std::fabs(c.u_z);
+  const double Dv = std::fabs(c.v_y) > std::fabs(c.v_x) ? std::fabs(c.v_y) : std::fabs(c.v_x);
+  c.h = (s * s * s) / (us * vs * Du * Dv);
+  c.Du = 0.5 * Du;
+  c.Dv = 0.5 * Dv;
+  c.us = us;
+  c.vs = vs;
+  c.nx = nx;
+  c.ny = ny;
+  c.nz = nz;
+  c.nu = nu;
+  c.nv = nv;
+  return c;
+}
+
+// One view of the distance-driven projector on host FP64 arrays: out_host +=
+// A in (job 0), the reference's transpose (job != 0, stuck) or the exact
+// adjoint (job != 0, !stuck), in the drop-in precision.
+static int dd_dropin(double *out, double *in, long nx, long ny, long nz, double s_scl, long nu,
+                     long nv, double u_scl, double v_scl, const double *xos, const double *xsd,
+                     int job, bool stuck) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!out || !in || !xos || !xsd) return fail("null argument");
+  if (bad_sizes(nx, ny, nz, nu, nv) || nx * ny * nz > (1L << 31) || nu * nv > (1L << 31))
+    return fail("invalid sizes");
+  if (!(s_scl > 0.0) || !(u_scl > 0.0) || !(v_scl > 0.0)) return fail("invalid scales");
+  if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return TRTT_FAILURE;
+  const trtt::DDConst c = dd_consts(nx, ny, nz, s_scl, nu, nv, u_scl, v_scl, xos, xsd);
+  if (!(c.h < HUGE_VAL)) return fail("degenerate projection (zero footprint size)");
+  const long nvol = nx * ny * nz, nsino = nu * nv;
+  const long nin = job == 0 ? nvol : nsino, nout = job == 0 ? nsino : nvol;
+  const bool f64 = g_precision == TRTT_HIP_F64;
+  const size_t esz = f64 ? 8 : 4;
+  if (g_scratch.device != g_device) {
+    for (int q = 0; q < 3; ++q) {
+      g_scratch.buf[q] = nullptr;
+      g_scratch.cap[q] = 0;
+    }
+    g_scratch.device = g_device;
+  }
+  if (!g_scratch.ensure(0, sizeof(double) * std::max(nin, nout)) ||
+      !g_scratch.ensure(1, esz * nin) || !g_scratch.ensure(2, esz * nout))
+    return TRTT_FAILURE;
+  double *d_stage = static_cast<double *>(g_scratch.buf[0]);
+  void *d_in = g_scratch.buf[1], *d_out = g_scratch.buf[2];
+  hipStream_t st = nullptr;
+  if (f64) {
+    if (!hip_ok(hipMemcpy(d_in, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
+      return TRTT_FAILURE;
+  } else {
+    if (!hip_ok(hipMemcpy(d_stage, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
+      return TRTT_FAILURE;
+    hipLaunchKernelGGL(cvt_d2f, dim3(grid_for(nin)), dim3(256), 0, st, d_stage,
+                       static_cast<float *>(d_in), nin);
+  }
+  if (!hip_ok(hipMemsetAsync(d_out, 0, esz * nout, st), "hipMemset") ||
+      !hip_ok(trtt::launch_dd(f64, c, job == 0 ? 0 : 1, stuck ? 1 : 0, d_in, d_out, st),
+              "kernel launch"))
+    return TRTT_FAILURE;
+  std::vector<double> res(nout);
+  if (f64) {
+    if (!hip_ok(hipMemcpy(res.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost),
+                "hipMemcpy"))
+      return TRTT_FAILURE;
+  } else {
+    hipLaunchKernelGGL(cvt_f2d, dim3(grid_for(nout)), dim3(256), 0, st,
+                       static_cast<const float *>(d_out), d_stage, nout);
+    if (!hip_ok(hipMemcpy(res.data(), d_stage, sizeof(double) * nout, hipMemcpyDeviceToHost),
+                "hipMemcpy"))
+      return TRTT_FAILURE;
+  }
+  for (long q = 0; q < nout; ++q) out[q] += res[q];
+  return TRTT_SUCCESS;
+}
+
+int trtt_DD_PB3D(double *out, double *in, long nx, long ny, long nz, double s_scl, long nu,
+                 long nv, double u_scl, double v_scl, double *xos, double *xsd, int job) {
+  return dd_dropin(out, in, nx, ny, nz, s_scl, nu, nv, u_scl, v_scl, xos, xsd, job, true);
+}
+
+int trtt_hip_DD_PB3D_adjoint(double *out, double *in, long nx, long ny, long nz, double s_scl,
+                             long nu, long nv, double u_scl, double v_scl, double *xos,
+                             double *xsd) {
+  return dd_dropin(out, in, nx, ny, nz, s_scl, nu, nv, u_scl, v_scl, xos, xsd, 1, false);
+}
+
 static int dropin(int kind, double *out, double *in, long nx, long ny, long nz, double s_scl,
                   int s_deg, const double *cum, long m, double step, const double *coord,
                   long nu, long nv, double u_scl, double v_scl, const double *xos,

@@ -61,6 +61,17 @@ struct DevPlan {
   double lf_vs, lf_ps, lf_R, lf_scale, lf_size;
 };
 
+// Constants of one view of the distance-driven parallel-beam projector
+// (trtt_DD_PB3D, trtt_3D_projectors.c:1077-1120), FP64, reference order.
+struct DDConst {
+  double v_x, v_y, v_z, v_c;  // detector v of voxel (i,j,k): ((v_z k + v_c) + v_y j) + v_x i
+  double u_x, u_y, u_z, u_c;
+  double Du, Dv;              // half sizes of the rectangular footprint (pixels)
+  double h;                   // s^3 / (u_scl v_scl |u_z| max(|v_x|, |v_y|))
+  double us, vs;
+  long nx, ny, nz, nu, nv;
+};
+
 // Launch shape of the axial kernels of a plan (trtt_hip_plan_set_tuning).
 constexpr int kNumFwdTiles = 3;  // 8x4, 4x4, 8x8 columns x 64 k
 constexpr int kNumBwdTiles = 6;  // 8x8, 8x4, 4x4 columns x 64 k; 8x4, 4x4, 8x8 x 128 k

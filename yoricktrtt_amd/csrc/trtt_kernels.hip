@@ -764,6 +764,119 @@ hipError_t launch_sparse_columns(int kind, bool table, bool quirk, bool f64, con
 }
 
 // ---------------------------------------------------------------------------
+// Distance-driven parallel-beam projector (trtt_DD_PB3D,
+// trtt_3D_projectors.c:1077-1173; burns :406-525 and :817-940).  A voxel's
+// footprint is the rectangle [uc - Du, uc + Du] x [vc - Dv, vc + Dv] with its
+// corners clamped to [0, n-1]; a pixel's weight is h times its overlap with
+// the rectangle along u (times u_scl) and along v (times v_scl).  One thread
+// per voxel: the forward scatters with atomics, the transpose gathers.
+// ---------------------------------------------------------------------------
+
+// Overlap of pixel floor(b1) + p with [b1, b2], scaled.  `stuck` gives the
+// reference transpose's v weights (:909-916): its loop over the inner pixels
+// advances the u counter, so all inner pixels write one slot, the upper
+// partial pixel overwrites it, and the slots after it stay zero.
+__device__ __forceinline__ double dd_overlap(double b1, double b2, int p, double scl, bool stuck) {
+#pragma clang fp contract(off)
+  const double up = ceil(b1), down = floor(b2);
+  const double inner = down - up;
+  const bool low = up > b1, high = down < b2;
+  if (inner < 0.0) return p == 0 ? scl * (b2 - b1) : 0.0;
+  if (!stuck) {
+    const int n = int(ceil(b2) - floor(b1));
+    if (low && p == 0) return scl * (up - b1);
+    if (high && p == n - 1) return scl * (b2 - down);
+    return scl;
+  }
+  if (low && p == 0) return scl * (up - b1);
+  if (p == (low ? 1 : 0)) return high ? scl * (b2 - down) : (inner > 0.0 ? scl : 0.0);
+  return 0.0;
+}
+
+struct DDRect {
+  double ub1, ub2, vb1, vb2;
+  int nu_e, nv_e;
+  long corner;
+};
+
+__device__ __forceinline__ void dd_rect(const DDConst &c, long i, long j, long k, DDRect &r) {
+#pragma clang fp contract(off)
+  const double vc = ((c.v_z * double(k) + c.v_c) + c.v_y * double(j)) + c.v_x * double(i);
+  const double uc = ((c.u_z * double(k) + c.u_c) + c.u_y * double(j)) + c.u_x * double(i);
+  const double nu1 = double(c.nu - 1), nv1 = double(c.nv - 1);
+  r.ub1 = fmin(fmax(uc - c.Du, 0.0), nu1);
+  r.ub2 = fmin(fmax(uc + c.Du, 0.0), nu1);
+  r.vb1 = fmin(fmax(vc - c.Dv, 0.0), nv1);
+  r.vb2 = fmin(fmax(vc + c.Dv, 0.0), nv1);
+  r.nu_e = int(ceil(r.ub2) - floor(r.ub1));
+  r.nv_e = int(ceil(r.vb2) - floor(r.vb1));
+  r.corner = long(floor(r.vb1)) * c.nu + long(floor(r.ub1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void dd_forward(DDConst c, const T *__restrict__ vol,
+                                                       T *__restrict__ sino) {
+#pragma clang fp contract(off)
+  const long vox = long(blockIdx.x) * kThreads + threadIdx.x;
+  if (vox >= c.nx * c.ny * c.nz) return;
+  const long i = vox % c.nx, j = (vox / c.nx) % c.ny, k = vox / (c.nx * c.ny);
+  DDRect r;
+  dd_rect(c, i, j, k, r);
+  if (r.nu_e <= 0 || r.nv_e <= 0) return;
+  const double x = double(vol[vox]);
+  for (int m = 0; m < r.nv_e; ++m) {
+    const double hv = c.h * dd_overlap(r.vb1, r.vb2, m, c.vs, false);
+    for (int l = 0; l < r.nu_e; ++l)
+      atomic_add(sino + r.corner + long(m) * c.nu + l,
+                 T(hv * dd_overlap(r.ub1, r.ub2, l, c.us, false) * x));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void dd_transpose(DDConst c, const T *__restrict__ sino,
+                                                         T *__restrict__ vol, int stuck) {
+#pragma clang fp contract(off)
+  const long vox = long(blockIdx.x) * kThreads + threadIdx.x;
+  if (vox >= c.nx * c.ny * c.nz) return;
+  const long i = vox % c.nx, j = (vox / c.nx) % c.ny, k = vox / (c.nx * c.ny);
+  DDRect r;
+  dd_rect(c, i, j, k, r);
+  if (r.nu_e <= 0 || r.nv_e <= 0) return;
+  double s = 0.0;
+  for (int m = 0; m < r.nv_e; ++m) {
+    const double hv = c.h * dd_overlap(r.vb1, r.vb2, m, c.vs, stuck != 0);
+    for (int l = 0; l < r.nu_e; ++l)
+      s += hv * dd_overlap(r.ub1, r.ub2, l, c.us, false) *
+           double(sino[r.corner + long(m) * c.nu + l]);
+  }
+  vol[vox] += T(s);
+}
+
+// job 0: sino += A vol; job 1: vol += the reference's transpose (stuck = 1)
+// or the exact adjoint (stuck = 0).
+hipError_t launch_dd(bool f64, const DDConst &c, int job, int stuck, const void *in, void *out,
+                     hipStream_t st) {
+  const long n = c.nx * c.ny * c.nz;
+  const dim3 grid(unsigned((n + kThreads - 1) / kThreads));
+  if (job == 0) {
+    if (f64)
+      hipLaunchKernelGGL(dd_forward<double>, grid, dim3(kThreads), 0, st, c,
+                         static_cast<const double *>(in), static_cast<double *>(out));
+    else
+      hipLaunchKernelGGL(dd_forward<float>, grid, dim3(kThreads), 0, st, c,
+                         static_cast<const float *>(in), static_cast<float *>(out));
+  } else {
+    if (f64)
+      hipLaunchKernelGGL(dd_transpose<double>, grid, dim3(kThreads), 0, st, c,
+                         static_cast<const double *>(in), static_cast<double *>(out), stuck);
+    else
+      hipLaunchKernelGGL(dd_transpose<float>, grid, dim3(kThreads), 0, st, c,
+                         static_cast<const float *>(in), static_cast<float *>(out), stuck);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers (explicit instantiation table)
 // ---------------------------------------------------------------------------
 template <typename T, int KIND, bool TABLE, bool QUIRK>
