@@ -59,3 +59,24 @@ def test_monotone_and_history():
     h = np.array(res.history)
     assert np.all(np.diff(h) <= 1e-12)
     assert res.niter == len(h) - 1
+
+
+def test_checkpoint_resume_continues_the_same_iterates(tmp_path):
+    """A run stopped after 7 iterations and resumed from its checkpoint ends
+    on the same iterate as an uninterrupted run of 18 (checkpointing has no
+    counterpart in the reference; OptimPack keeps no state on disk)."""
+    from yoricktrtt_amd.vmlmb import load_checkpoint
+
+    A, b, fg = _quadratic(50, 4)
+    x0 = torch.full((50,), 0.2, dtype=torch.float64)
+    full = vmlmb(fg, x0, lower=0.0, mem=5, maxiter=18, grtol=1e-3)
+    ck = tmp_path / "state.npz"
+    part = vmlmb(fg, x0, lower=0.0, mem=5, maxiter=7, grtol=1e-3, checkpoint=ck,
+                 checkpoint_every=1)
+    assert part.niter == 7
+    x, pairs, niter, f, g0 = load_checkpoint(ck)
+    assert niter == 7 and f == part.f and len(pairs) == 5
+    assert torch.equal(x, part.x)
+    rest = vmlmb(fg, x, lower=0.0, mem=5, maxiter=18 - niter, grtol=1e-3, pairs=pairs, g0norm=g0)
+    assert torch.equal(rest.x, full.x)
+    assert rest.history[1:] == full.history[8:]

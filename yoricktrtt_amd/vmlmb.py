@@ -36,6 +36,42 @@ class Result:
     history: list = field(default_factory=list)
     pairs: list = field(default_factory=list)  # L-BFGS memory (s, y, 1/s'y) at exit
     grad: object = None  # gradient at x
+    g0norm: float = None  # projected gradient norm of the first iteration (grtol scale)
+
+
+def save_checkpoint(path, x, pairs, niter, f, g0norm=None):
+    """Solver state after an iteration: the iterate, the L-BFGS memory, the
+    iteration count, f and the grtol scale, as an .npz archive (plain
+    arrays, no pickled objects)."""
+    import numpy as np
+
+    arrays = {"x": x.detach().cpu().numpy(), "niter": np.array(niter), "f": np.array(f),
+              "g0norm": np.array(np.nan if g0norm is None else g0norm),
+              "npairs": np.array(len(pairs))}
+    for k, (s, y, rho) in enumerate(pairs):
+        arrays[f"s{k}"] = s.detach().cpu().numpy()
+        arrays[f"y{k}"] = y.detach().cpu().numpy()
+        arrays[f"rho{k}"] = np.array(rho)
+    tmp = str(path) + ".tmp.npz"
+    np.savez(tmp, **arrays)
+    import os
+
+    os.replace(tmp, str(path))
+
+
+def load_checkpoint(path, device=None):
+    """(x, pairs, niter, f, g0norm) of a checkpoint written by save_checkpoint,
+    with tensors on ``device``."""
+    import numpy as np
+    import torch
+
+    with np.load(str(path), allow_pickle=False) as z:
+        x = torch.from_numpy(z["x"].copy()).to(device)
+        pairs = [(torch.from_numpy(z[f"s{k}"].copy()).to(device),
+                  torch.from_numpy(z[f"y{k}"].copy()).to(device), float(z[f"rho{k}"]))
+                 for k in range(int(z["npairs"]))]
+        g0 = float(z["g0norm"])
+        return x, pairs, int(z["niter"]), float(z["f"]), (None if math.isnan(g0) else g0)
 
 
 def _project(x, lower, upper):
@@ -47,7 +83,8 @@ def _project(x, lower, upper):
 
 
 def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol=0.0,
-          grtol=0.0, frtol=0.0, ftol=1e-4, fstop=None, callback=None, pairs=None):
+          grtol=0.0, frtol=0.0, ftol=1e-4, fstop=None, callback=None, pairs=None, g0norm=None,
+          checkpoint=None, checkpoint_every=1):
     """Minimise f over lower <= x <= upper.
 
     ``fg(x, g)`` returns f(x) as a float and writes the gradient into g.
@@ -57,6 +94,10 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
     ``callback(it, x, f)`` is called after each iteration.  ``pairs`` seeds
     the L-BFGS memory (e.g. the ``pairs`` of an earlier run on the same
     objective, so a restart from a new point keeps its curvature model).
+    ``checkpoint`` names a file that receives the solver state every
+    ``checkpoint_every`` iterations (save_checkpoint); a run started from a
+    checkpoint's x, pairs and g0norm (load_checkpoint) continues with the
+    same iterates as the uninterrupted run.
     """
     import torch
 
@@ -66,7 +107,6 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
     neval = 1
     pairs = list(pairs[-mem:]) if pairs else []  # (s, y, rho)
     history = [f]
-    g0norm = None
     status = "maxiter"
     xn = torch.empty_like(x)
     gn = torch.empty_like(x)
@@ -143,6 +183,8 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
         it += 1
         if callback is not None:
             callback(it, x, f)
+        if checkpoint is not None and it % checkpoint_every == 0:
+            save_checkpoint(checkpoint, x, pairs, it, f, g0norm)
         if frtol > 0.0 and abs(fprev - f) <= frtol * max(abs(f), abs(fprev)):
             status = "frtol"
             break
@@ -153,4 +195,4 @@ def vmlmb(fg, x0, lower=None, upper=None, mem=5, maxiter=50, maxeval=None, gatol
             status = "maxeval"
             break
     return Result(x=x, f=f, niter=it, neval=neval, status=status, history=history, pairs=pairs,
-                  grad=g)
+                  grad=g, g0norm=g0norm)
