@@ -253,6 +253,20 @@ int trtt_hip_sparse_forward(trtt_hip_sparse_t *m, const void *img, void *sino, v
 int trtt_hip_sparse_backward(trtt_hip_sparse_t *m, const void *sino, void *img, void *stream);
 int trtt_hip_sparse_destroy(trtt_hip_sparse_t *m);
 
+/* Distance-driven parallel-beam projector over many views on the device
+ * (the batched form of trtt_DD_PB3D; no reference counterpart): views k <
+ * nviews of xos/xsd (nviews*12 each), volume [nz][ny][nx], sinogram
+ * [nviews][nv][nu], plan precision.  backward: exact = 1 gives the adjoint of
+ * forward, exact = 0 the reference's transpose (trtt_3D_projectors.c:
+ * 909-916).  Asynchronous on `stream`; outputs accumulate. */
+typedef struct trtt_hip_dd trtt_hip_dd_t;
+trtt_hip_dd_t *trtt_hip_dd_create(int precision, long nx, long ny, long nz, double s_scl,
+                                  long nu, long nv, double u_scl, double v_scl, const double *xos,
+                                  const double *xsd, long nviews);
+int trtt_hip_dd_forward(trtt_hip_dd_t *p, const void *vol, void *sino, void *stream);
+int trtt_hip_dd_backward(trtt_hip_dd_t *p, const void *sino, void *vol, int exact, void *stream);
+int trtt_hip_dd_destroy(trtt_hip_dd_t *p);
+
 /* Temporal interpolation of a series of nt frames (images or volumes of npix
  * values, plan precision): the B_k operators of the dynamic systems,
  * _trtt_2D_time_interpolator (Ytrtt_2D_projectors.i:761-814) and the

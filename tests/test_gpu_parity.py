@@ -147,6 +147,41 @@ def test_dropin_dd_pb3d_vs_reference(lib, oracle, precision, name):
         assert rel_err(a, ra) < TOL[precision]
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dd_batched_vs_oracle(lib, oracle, prec):
+    """Batched distance-driven projector over 9 views (pixels smaller than
+    the voxels, so the reference transpose differs from the adjoint)."""
+    from yoricktrtt_amd.geometry import parallel_geometry_3d
+    from yoricktrtt_amd.projectors import DDProjector
+
+    _say("dd batched", prec)
+    nx, ny, nz, nu, nv, V = 21, 19, 15, 44, 50, 9
+    th = np.arange(V) * (math.pi / V) + 0.2
+    geo = parallel_geometry_3d(nx, ny, nz, 1.0, nu, nv, 0.55, 0.5, th)
+    P = DDProjector(geo, precision=prec)
+    rng = np.random.default_rng(9)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    dt = torch.float64 if prec == "f64" else torch.float32
+    fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
+    adj = P.backward(_dev(sino, dt)).double().cpu().numpy()
+    ref_t = P.backward(_dev(sino, dt), exact=False).double().cpu().numpy()
+    ra = np.zeros((nz, ny, nx))
+    rt = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.dd_pb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 0.55, 0.5, geo.xos[v],
+                            geo.xsd[v], 0)
+        assert rel_err(fwd[v], rf) < TOL[prec], v
+        oracle.dd_pb3d(ra, sino[v], nx, ny, nz, 1.0, nu, nv, 0.55, 0.5, geo.xos[v], geo.xsd[v], 1,
+                       adjoint=True)
+        oracle.dd_pb3d(rt, sino[v], nx, ny, nz, 1.0, nu, nv, 0.55, 0.5, geo.xos[v], geo.xsd[v], 1)
+    assert rel_err(adj, ra) < TOL[prec]
+    assert rel_err(ref_t, rt) < TOL[prec]
+    a, b = float((fwd * sino).sum()), float((vol * adj).sum())
+    assert abs(a - b) <= (1e-12 if prec == "f64" else 1e-6) * abs(a)
+    P.close()
+
+
 def test_dropin_accumulates(lib, golden3d):
     """The drop-ins add into `out` like the reference (out += A x)."""
     g = golden3d

@@ -80,6 +80,10 @@ _SIGS = {
     "trtt_hip_tinterp_create": ([I, L, L, L, I, ip, dp], vp),
     "trtt_hip_tinterp_apply": ([vp, I, vp, vp, vp], I),
     "trtt_hip_tinterp_destroy": ([vp], I),
+    "trtt_hip_dd_create": ([I, L, L, L, D, L, L, D, D, dp, dp, L], vp),
+    "trtt_hip_dd_forward": ([vp, vp, vp, vp], I),
+    "trtt_hip_dd_backward": ([vp, vp, vp, I, vp], I),
+    "trtt_hip_dd_destroy": ([vp], I),
     "trtt_hip_tv": ([vp, vp, vp, vp, L, L, L, L, D, D, D, I, vp], I),
     "trtt_hip_bspline_filter": ([vp, vp, L, L, L, L, I, I, vp], I),
     "trtt_unet_conv_fwd": ([vp, vp, vp, vp, vp, I, I, I, I, I, I, I, I, I, vp], I),

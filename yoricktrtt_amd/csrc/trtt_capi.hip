@@ -39,8 +39,8 @@ hipError_t sparse_compress(bool f64, const unsigned *key_sorted, const unsigned 
                            int *ptr, long n, int n_major, hipStream_t st);
 hipError_t sparse_spmv(bool f64, const int *ptr, const int *idx, const void *val, const void *x,
                        void *y, int n_major, hipStream_t st);
-hipError_t launch_dd(bool f64, const DDConst &c, int job, int stuck, const void *in, void *out,
-                     hipStream_t st);
+hipError_t launch_dd(bool f64, const DDConst *cs, const DDConst &c0, int nviews, int job,
+                     int stuck, const void *in, void *out, hipStream_t st);
 hipError_t launch_tinterp(bool f64, int job, const void *in, void *out, long npix, int nframes,
                           int taps, const int *idx, const void *w, const int *ptr,
                           const int *src, const void *wb, hipStream_t st);
@@ -389,8 +389,8 @@ int views_per_block(const trtt_hip_plan *p) {
 // Scratch buffers of the synchronous drop-in symbols (grow-only, per process)
 // ---------------------------------------------------------------------------
 struct Scratch {
-  void *buf[3] = {nullptr, nullptr, nullptr};
-  size_t cap[3] = {0, 0, 0};
+  void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t cap[4] = {0, 0, 0, 0};
   int device = -1;
   bool ensure(int slot, size_t bytes) {
     if (cap[slot] >= bytes) return true;
@@ -430,7 +430,7 @@ int run_dropin(trtt_hip_plan *p, double *out, const double *in, int job) {
   const bool f64 = p->precision == TRTT_HIP_F64;
   const size_t esz = f64 ? 8 : 4;
   if (g_scratch.device != p->device) {
-    for (int s = 0; s < 3; ++s) {
+    for (int s = 0; s < 4; ++s) {
       g_scratch.buf[s] = nullptr;
       g_scratch.cap[s] = 0;
     }
@@ -1240,18 +1240,22 @@ static int dd_dropin(double *out, double *in, long nx, long ny, long nz, double 
   const bool f64 = g_precision == TRTT_HIP_F64;
   const size_t esz = f64 ? 8 : 4;
   if (g_scratch.device != g_device) {
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
       g_scratch.buf[q] = nullptr;
       g_scratch.cap[q] = 0;
     }
     g_scratch.device = g_device;
   }
   if (!g_scratch.ensure(0, sizeof(double) * std::max(nin, nout)) ||
-      !g_scratch.ensure(1, esz * nin) || !g_scratch.ensure(2, esz * nout))
+      !g_scratch.ensure(1, esz * nin) || !g_scratch.ensure(2, esz * nout) ||
+      !g_scratch.ensure(3, sizeof(trtt::DDConst)))
     return TRTT_FAILURE;
   double *d_stage = static_cast<double *>(g_scratch.buf[0]);
   void *d_in = g_scratch.buf[1], *d_out = g_scratch.buf[2];
+  trtt::DDConst *d_c = static_cast<trtt::DDConst *>(g_scratch.buf[3]);
   hipStream_t st = nullptr;
+  if (!hip_ok(hipMemcpy(d_c, &c, sizeof(c), hipMemcpyHostToDevice), "hipMemcpy"))
+    return TRTT_FAILURE;
   if (f64) {
     if (!hip_ok(hipMemcpy(d_in, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
       return TRTT_FAILURE;
@@ -1262,7 +1266,7 @@ static int dd_dropin(double *out, double *in, long nx, long ny, long nz, double 
                        static_cast<float *>(d_in), nin);
   }
   if (!hip_ok(hipMemsetAsync(d_out, 0, esz * nout, st), "hipMemset") ||
-      !hip_ok(trtt::launch_dd(f64, c, job == 0 ? 0 : 1, stuck ? 1 : 0, d_in, d_out, st),
+      !hip_ok(trtt::launch_dd(f64, d_c, c, 1, job == 0 ? 0 : 1, stuck ? 1 : 0, d_in, d_out, st),
               "kernel launch"))
     return TRTT_FAILURE;
   std::vector<double> res(nout);
@@ -1290,6 +1294,79 @@ int trtt_hip_DD_PB3D_adjoint(double *out, double *in, long nx, long ny, long nz,
                              long nu, long nv, double u_scl, double v_scl, double *xos,
                              double *xsd) {
   return dd_dropin(out, in, nx, ny, nz, s_scl, nu, nv, u_scl, v_scl, xos, xsd, 1, false);
+}
+
+}  // extern "C"
+
+// Batched distance-driven projector: per-view constants on the device.
+struct trtt_hip_dd {
+  int device = 0;
+  bool f64 = false;
+  int nviews = 0;
+  trtt::DDConst c0{};
+  trtt::DDConst *d_c = nullptr;
+};
+
+extern "C" {
+
+trtt_hip_dd_t *trtt_hip_dd_create(int precision, long nx, long ny, long nz, double s_scl,
+                                  long nu, long nv, double u_scl, double v_scl, const double *xos,
+                                  const double *xsd, long nviews) {
+  if (!xos || !xsd) {
+    fail("null argument");
+    return nullptr;
+  }
+  if ((precision != TRTT_HIP_F32 && precision != TRTT_HIP_F64) || bad_sizes(nx, ny, nz, nu, nv) ||
+      nx * ny * nz > (1L << 31) || nu * nv > (1L << 31) || nviews <= 0 || nviews > 65535 ||
+      !(s_scl > 0.0) || !(u_scl > 0.0) || !(v_scl > 0.0)) {
+    fail("invalid distance-driven plan arguments");
+    return nullptr;
+  }
+  std::vector<trtt::DDConst> cs(nviews);
+  for (long k = 0; k < nviews; ++k) {
+    cs[k] = dd_consts(nx, ny, nz, s_scl, nu, nv, u_scl, v_scl, xos + 12 * k, xsd + 12 * k);
+    if (!(cs[k].h < HUGE_VAL)) {
+      fail("degenerate projection (zero footprint size)");
+      return nullptr;
+    }
+  }
+  if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return nullptr;
+  trtt_hip_dd *p = new trtt_hip_dd;
+  p->device = g_device;
+  p->f64 = precision == TRTT_HIP_F64;
+  p->nviews = int(nviews);
+  p->c0 = cs[0];
+  if (!upload_vec(cs, p->d_c)) {
+    delete p;
+    return nullptr;
+  }
+  return p;
+}
+
+int trtt_hip_dd_forward(trtt_hip_dd_t *p, const void *vol, void *sino, void *stream) {
+  if (!p || !vol || !sino) return fail("null argument");
+  return hip_ok(trtt::launch_dd(p->f64, p->d_c, p->c0, p->nviews, 0, 0, vol, sino,
+                                static_cast<hipStream_t>(stream)),
+                "distance-driven forward")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_dd_backward(trtt_hip_dd_t *p, const void *sino, void *vol, int exact, void *stream) {
+  if (!p || !vol || !sino) return fail("null argument");
+  return hip_ok(trtt::launch_dd(p->f64, p->d_c, p->c0, p->nviews, 1, exact ? 0 : 1, sino, vol,
+                                static_cast<hipStream_t>(stream)),
+                "distance-driven backward")
+             ? TRTT_SUCCESS
+             : TRTT_FAILURE;
+}
+
+int trtt_hip_dd_destroy(trtt_hip_dd_t *p) {
+  if (!p) return TRTT_SUCCESS;
+  hipSetDevice(p->device);
+  hipFree(p->d_c);
+  delete p;
+  return TRTT_SUCCESS;
 }
 
 static int dropin(int kind, double *out, double *in, long nx, long ny, long nz, double s_scl,

@@ -813,64 +813,76 @@ __device__ __forceinline__ void dd_rect(const DDConst &c, long i, long j, long k
   r.corner = long(floor(r.vb1)) * c.nu + long(floor(r.ub1));
 }
 
+// Views of a batch: blockIdx.y selects view `view0 + blockIdx.y` of `cs` in
+// the forward; the transpose loops over the views in each thread.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void dd_forward(DDConst c, const T *__restrict__ vol,
+__global__ __launch_bounds__(kThreads) void dd_forward(const DDConst *__restrict__ cs,
+                                                       const T *__restrict__ vol,
                                                        T *__restrict__ sino) {
 #pragma clang fp contract(off)
+  const DDConst &c = cs[blockIdx.y];
   const long vox = long(blockIdx.x) * kThreads + threadIdx.x;
   if (vox >= c.nx * c.ny * c.nz) return;
   const long i = vox % c.nx, j = (vox / c.nx) % c.ny, k = vox / (c.nx * c.ny);
   DDRect r;
   dd_rect(c, i, j, k, r);
   if (r.nu_e <= 0 || r.nv_e <= 0) return;
+  T *out = sino + long(blockIdx.y) * c.nu * c.nv;
   const double x = double(vol[vox]);
   for (int m = 0; m < r.nv_e; ++m) {
     const double hv = c.h * dd_overlap(r.vb1, r.vb2, m, c.vs, false);
     for (int l = 0; l < r.nu_e; ++l)
-      atomic_add(sino + r.corner + long(m) * c.nu + l,
+      atomic_add(out + r.corner + long(m) * c.nu + l,
                  T(hv * dd_overlap(r.ub1, r.ub2, l, c.us, false) * x));
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void dd_transpose(DDConst c, const T *__restrict__ sino,
+__global__ __launch_bounds__(kThreads) void dd_transpose(const DDConst *__restrict__ cs,
+                                                         int nviews, const T *__restrict__ sino,
                                                          T *__restrict__ vol, int stuck) {
 #pragma clang fp contract(off)
   const long vox = long(blockIdx.x) * kThreads + threadIdx.x;
-  if (vox >= c.nx * c.ny * c.nz) return;
-  const long i = vox % c.nx, j = (vox / c.nx) % c.ny, k = vox / (c.nx * c.ny);
-  DDRect r;
-  dd_rect(c, i, j, k, r);
-  if (r.nu_e <= 0 || r.nv_e <= 0) return;
+  if (vox >= cs[0].nx * cs[0].ny * cs[0].nz) return;
+  const long i = vox % cs[0].nx, j = (vox / cs[0].nx) % cs[0].ny, k = vox / (cs[0].nx * cs[0].ny);
   double s = 0.0;
-  for (int m = 0; m < r.nv_e; ++m) {
-    const double hv = c.h * dd_overlap(r.vb1, r.vb2, m, c.vs, stuck != 0);
-    for (int l = 0; l < r.nu_e; ++l)
-      s += hv * dd_overlap(r.ub1, r.ub2, l, c.us, false) *
-           double(sino[r.corner + long(m) * c.nu + l]);
+  for (int view = 0; view < nviews; ++view) {
+    const DDConst &c = cs[view];
+    DDRect r;
+    dd_rect(c, i, j, k, r);
+    if (r.nu_e <= 0 || r.nv_e <= 0) continue;
+    const T *in = sino + long(view) * c.nu * c.nv;
+    for (int m = 0; m < r.nv_e; ++m) {
+      const double hv = c.h * dd_overlap(r.vb1, r.vb2, m, c.vs, stuck != 0);
+      for (int l = 0; l < r.nu_e; ++l)
+        s += hv * dd_overlap(r.ub1, r.ub2, l, c.us, false) *
+             double(in[r.corner + long(m) * c.nu + l]);
+    }
   }
   vol[vox] += T(s);
 }
 
-// job 0: sino += A vol; job 1: vol += the reference's transpose (stuck = 1)
-// or the exact adjoint (stuck = 0).
-hipError_t launch_dd(bool f64, const DDConst &c, int job, int stuck, const void *in, void *out,
-                     hipStream_t st) {
-  const long n = c.nx * c.ny * c.nz;
-  const dim3 grid(unsigned((n + kThreads - 1) / kThreads));
+// Every view of `cs` (device array, nviews entries, same sizes): job 0:
+// sino[view] += A_view vol; job 1: vol += sum over the views of the
+// reference's transpose (stuck = 1) or of the exact adjoint (stuck = 0).
+hipError_t launch_dd(bool f64, const DDConst *cs, const DDConst &c0, int nviews, int job,
+                     int stuck, const void *in, void *out, hipStream_t st) {
+  const long n = c0.nx * c0.ny * c0.nz;
+  const unsigned blocks = unsigned((n + kThreads - 1) / kThreads);
   if (job == 0) {
+    const dim3 grid(blocks, unsigned(nviews));
     if (f64)
-      hipLaunchKernelGGL(dd_forward<double>, grid, dim3(kThreads), 0, st, c,
+      hipLaunchKernelGGL(dd_forward<double>, grid, dim3(kThreads), 0, st, cs,
                          static_cast<const double *>(in), static_cast<double *>(out));
     else
-      hipLaunchKernelGGL(dd_forward<float>, grid, dim3(kThreads), 0, st, c,
+      hipLaunchKernelGGL(dd_forward<float>, grid, dim3(kThreads), 0, st, cs,
                          static_cast<const float *>(in), static_cast<float *>(out));
   } else {
     if (f64)
-      hipLaunchKernelGGL(dd_transpose<double>, grid, dim3(kThreads), 0, st, c,
+      hipLaunchKernelGGL(dd_transpose<double>, dim3(blocks), dim3(kThreads), 0, st, cs, nviews,
                          static_cast<const double *>(in), static_cast<double *>(out), stuck);
     else
-      hipLaunchKernelGGL(dd_transpose<float>, grid, dim3(kThreads), 0, st, c,
+      hipLaunchKernelGGL(dd_transpose<float>, dim3(blocks), dim3(kThreads), 0, st, cs, nviews,
                          static_cast<const float *>(in), static_cast<float *>(out), stuck);
   }
   return hipGetLastError();

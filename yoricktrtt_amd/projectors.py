@@ -600,3 +600,70 @@ class DynamicProjector:
         self.P.close()
         if self.B is not None:
             self.B.close()
+
+
+class DDProjector:
+    """Distance-driven parallel-beam projector over every view of a
+    ``parallel3d`` geometry on the device (batched trtt_DD_PB3D,
+    trtt_3D_projectors.c:1077-1173).  ``backward(..., exact=True)`` is the
+    adjoint of ``forward``; ``exact=False`` reproduces the reference's
+    transpose, which is not (trtt_3D_projectors.c:909-916)."""
+
+    def __init__(self, geom: Geometry, precision: str = "f32"):
+        import torch
+
+        if geom.kind != "parallel3d":
+            raise ValueError("the distance-driven projector is parallel-beam 3-D")
+        self.torch = torch
+        self.geom = geom
+        self.lib = _lib.load()
+        self.precision = _lib.F64 if precision == "f64" else _lib.F32
+        self.dtype = torch.float64 if self.precision == _lib.F64 else torch.float32
+        xos, xsd = _f64(geom.xos), _f64(geom.xsd)
+        self.handle = self.lib.trtt_hip_dd_create(self.precision, geom.nx, geom.ny, geom.nz,
+                                                  geom.s_scl, geom.nu, geom.nv, geom.u_scl,
+                                                  geom.v_scl, _ptr(xos), _ptr(xsd), geom.nviews)
+        if not self.handle:
+            raise RuntimeError("distance-driven plan creation failed: " + _lib.last_error())
+
+    @property
+    def vol_shape(self):
+        g = self.geom
+        return (g.nz, g.ny, g.nx)
+
+    @property
+    def sino_shape(self):
+        g = self.geom
+        return (g.nviews, g.nv, g.nu)
+
+    def _stream(self, stream):
+        s = stream if stream is not None else self.torch.cuda.current_stream()
+        return C.c_void_p(s.cuda_stream)
+
+    def forward(self, vol, sino=None, stream=None):
+        if sino is None:
+            sino = self.torch.zeros(self.sino_shape, dtype=self.dtype, device=vol.device)
+        if self.lib.trtt_hip_dd_forward(self.handle, C.c_void_p(vol.data_ptr()),
+                                        C.c_void_p(sino.data_ptr()), self._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return sino
+
+    def backward(self, sino, vol=None, exact: bool = True, stream=None):
+        if vol is None:
+            vol = self.torch.zeros(self.vol_shape, dtype=self.dtype, device=sino.device)
+        if self.lib.trtt_hip_dd_backward(self.handle, C.c_void_p(sino.data_ptr()),
+                                         C.c_void_p(vol.data_ptr()), int(exact),
+                                         self._stream(stream)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return vol
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.trtt_hip_dd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
