@@ -194,3 +194,27 @@ def test_sparse_rejects_unsupported_plans(lib):
     assert "decimals" in _lib.last_error()
     P2.close()
     P3.close()
+
+
+def test_sparse_matrix_without_entries(lib):
+    """A detector that no footprint reaches: an empty matrix whose products
+    leave their outputs unchanged."""
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("sparse empty")
+    # parallel beam with the detector shifted far off the object
+    geo = _geometry("parallel2d", 12, 10, 8, 3, voff=np.array([200.0, 200.0, 200.0]))
+    P = Projector(geo, model="table", precision="f64")
+    M = P.sparse_matrix(6)
+    assert M.nnz == 0
+    coefs, rows, cols = M.coo()
+    assert coefs.size == rows.size == cols.size == 0
+    x = torch.rand(P.vol_shape, dtype=torch.float64, device="cuda:0")
+    y = torch.full(P.sino_shape, 2.0, dtype=torch.float64, device="cuda:0")
+    M.forward(x, y)
+    assert torch.all(y == 2.0)
+    g = torch.full(P.vol_shape, 3.0, dtype=torch.float64, device="cuda:0")
+    M.backward(y, g)
+    assert torch.all(g == 3.0)
+    M.close()
+    P.close()
