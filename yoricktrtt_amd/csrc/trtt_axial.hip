@@ -510,10 +510,10 @@ struct ColLane {
   bool bad;     // wk == 0 (voxel in the source plane)
 };
 
-template <bool CONE, int CX, int CY, int NCOL, typename T>
-__device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &g, const TileMap &tm,
-                                             int wave, int lane, int kf, int kl, ColLane<T> &cl) {
-  const int col = wave * NCOL + lane;
+// Column `col` of the tile on view g; `active` false leaves the lane empty.
+template <bool CONE, int CX, int CY, typename T>
+__device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &g, const TileMap &tm,
+                                             int col, bool active, int kf, int kl, ColLane<T> &cl) {
   cl.i = tm.bx * CX + col % CX;
   cl.j = tm.by * CY + col / CX;
   cl.base = (cl.j * P.nx + cl.i) * P.nz;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &
   cl.bad = false;
   cl.lo = cl.hi = 0;
   cl.np = 0;
-  if (lane < NCOL && cl.i < P.nx && cl.j < P.ny) {
+  if (active && cl.i < P.nx && cl.j < P.ny) {
     column<CONE>(P, g, cl.i, cl.j, cl.c);
     cl.bad = !cl.c.ok;
     if (cl.c.ok && cl.c.vl < cl.c.vr) {
@@ -529,6 +529,12 @@ __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &
       cl.live = cl.lo < cl.hi;
     }
   }
+}
+
+template <bool CONE, int CX, int CY, int NCOL, typename T>
+__device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &g, const TileMap &tm,
+                                             int wave, int lane, int kf, int kl, ColLane<T> &cl) {
+  column_entry<CONE, CX, CY>(P, g, tm, wave * NCOL + lane, lane < NCOL, kf, kl, cl);
 }
 
 __device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
@@ -636,8 +642,21 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
   prof[lane] = T(0);
   prof[lane + 64] = T(0);
 
+  // When the column tables of all views of the block fit in the wave, they
+  // are evaluated at once: lane vv NCOL + c holds column c on view vv.
+  const bool multi = views_per_block * NCOL <= 64;
+  const int first_view = view_base + int(blockIdx.y) * views_per_block;
+  ColLane<T> cl;
+  if (multi) {
+    const int slot = lane / NCOL;
+    const bool active = slot < views_per_block && first_view + slot < view_end;
+    column_entry<CONE, CX, CY>(P, P.views[active ? first_view + slot : first_view], tm,
+                               lane % NCOL, active, kf, kl, cl);
+    if (cl.bad) atomicOr(P.err, 1);
+  }
+
   for (int vv = 0; vv < views_per_block; ++vv) {
-    const int view = view_base + blockIdx.y * views_per_block + vv;
+    const int view = first_view + vv;
     if (view >= view_end) break;
     const ViewConst &g = P.views[view];
     T *out = sino + long(view) * npix;
@@ -650,16 +669,20 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     const T *vol0 = vol + long(g.t0) * P.vox_per_phase;
     const T *vol1 = vol + long(g.t1) * P.vox_per_phase;
 
-    ColLane<T> cl;
-    column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
-    if (cl.bad) atomicOr(P.err, 1);
+    if (!multi) {
+      column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
+      if (cl.bad) atomicOr(P.err, 1);
+    }
+    // this view's columns sit in lanes off .. off + NCOL - 1
+    const int off = multi ? vv * NCOL : 0;
+    const bool live = cl.live && lane >= off && lane < off + NCOL;
     // wave-uniform (readfirstlane: kept in scalar registers)
     // the window starts on a multiple of 4 pixels and its rows are a
     // multiple of 4 long (16-byte loads in the reduction)
-    const int ul = bcast_i(wred_min(cl.live ? cl.lo : INT_MAX), 0) & ~3;
-    const int ur = bcast_i(wred_max(cl.live ? cl.hi : INT_MIN), 0);
-    const int vl = bcast_i(wred_min(cl.live ? cl.c.vl : INT_MAX), 0);
-    const int vr = bcast_i(wred_max(cl.live ? cl.c.vr : INT_MIN), 0);
+    const int ul = bcast_i(wred_min(live ? cl.lo : INT_MAX), 0) & ~3;
+    const int ur = bcast_i(wred_max(live ? cl.hi : INT_MIN), 0);
+    const int vl = bcast_i(wred_min(live ? cl.c.vl : INT_MAX), 0);
+    const int vr = bcast_i(wred_max(live ? cl.c.vr : INT_MIN), 0);
     const bool any = ur > ul && vr > vl;
     const int RW = any ? (ur - ul + 3) & ~3 : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && RW * RH <= wcap;
@@ -680,7 +703,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       // the voxels of the next live column are loaded while the current one
       // is projected (volume copy with k fastest: the 64 lanes read 64
       // consecutive words)
-      unsigned long long pend = __ballot(cl.live);
+      unsigned long long pend = __ballot(live);
       T xa = T(0), xb = T(0);
       auto load_col = [&](int col) {
         const unsigned idx = unsigned(bcast_i(cl.base, col) + kc);
