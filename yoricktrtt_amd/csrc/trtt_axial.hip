@@ -576,12 +576,12 @@ __device__ __forceinline__ int wred_max(int x) {
 
 // Block window: union over the live columns of the four waves.
 template <typename T>
-__device__ __forceinline__ void block_window(const ColLane<T> &cl, int *red, int &ul, int &ur,
-                                             int &vl, int &vr) {
-  ul = wred_min(cl.live ? cl.lo : INT_MAX);
-  ur = wred_max(cl.live ? cl.hi : INT_MIN);
-  vl = wred_min(cl.live ? cl.c.vl : INT_MAX);
-  vr = wred_max(cl.live ? cl.c.vr : INT_MIN);
+__device__ __forceinline__ void block_window(const ColLane<T> &cl, bool live, int *red, int &ul,
+                                             int &ur, int &vl, int &vr) {
+  ul = wred_min(live ? cl.lo : INT_MAX);
+  ur = wred_max(live ? cl.hi : INT_MIN);
+  vl = wred_min(live ? cl.c.vl : INT_MAX);
+  vr = wred_max(live ? cl.c.vr : INT_MIN);
   block_union(ul, ur, vl, vr, red);
 }
 
@@ -925,17 +925,31 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 #pragma unroll
     for (int a = 0; a < (TWO ? NA : 1); ++a) acc1[a] = T(0);
 
+    // The column tables of NVT consecutive views of the segment are
+    // evaluated at once: lane v NCOL + c holds column c on view n0 + v.
+    constexpr int NVT = 64 / NCOL;
+    static_assert(64 % NCOL == 0, "views per column table");
+    ColLane<T> cl;
     for (int n = 0; n < seg.count; ++n) {
+      const int vslot = n % NVT;
+      if (vslot == 0) {
+        const int s2 = lane / NCOL;
+        const bool active = n + s2 < seg.count;
+        const int vw = P.order[seg.start + (active ? n + s2 : n)];
+        column_entry<CONE, CX, CY>(P, P.views[vw], tm, wave * NCOL + lane % NCOL, active, kf, kl,
+                                   cl);
+        if (cl.bad) atomicOr(P.err, 1);
+      }
       const int view = P.order[seg.start + n];
       const ViewConst &g = P.views[view];
       const T *in = sino + long(view) * npix;
       const double a12 = g.a12;
       const T w0 = T(g.w0), w1 = T(g.w1), wsum = T(g.w0 + g.w1);
-      ColLane<T> cl;
-      column_table<CONE, CX, CY, NCOL>(P, g, tm, wave, lane, kf, kl, cl);
-      if (cl.bad) atomicOr(P.err, 1);
+      // this view's columns sit in lanes off .. off + NCOL - 1
+      const int off = vslot * NCOL;
+      const bool live = cl.live && lane >= off && lane < off + NCOL;
       int ul, ur, vl, vr;
-      block_window(cl, red, ul, ur, vl, vr);
+      block_window(cl, live, red, ul, ur, vl, vr);
       // wave-uniform (readfirstlane: kept in scalar registers)
       ul = bcast_i(ul, 0);
       ur = bcast_i(ur, 0);
@@ -963,13 +977,13 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       }
       __syncthreads();
       if (any) {
-        const unsigned long long live_mask = __ballot(cl.live);
+        const unsigned long long live_mask = __ballot(live) >> off;
 #pragma unroll 1
         for (int cc = 0; cc < NCOL; ++cc) {
           if (!((live_mask >> cc) & 1ull)) continue;
           Col<T> c;
           int lo, hi, np, i, j;
-          bcast_col(cl, cc, c, lo, hi, np, i, j);
+          bcast_col(cl, off + cc, c, lo, hi, np, i, j);
           np = min(np, W);
           const int span = hi - lo;
           const int npv = c.vr - c.vl;
