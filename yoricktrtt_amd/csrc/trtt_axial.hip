@@ -32,6 +32,8 @@ namespace axial {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kKT = 64;  // k per tile (one per lane)
+// fixed row pitch of the backward's staged window (one row per thread)
+constexpr int kPitch = kThreads;
 
 // F(x) - 1/2 for the cubic B-spline CDF: sign(x) G(min(|x|, 2)) with, on
 // [0, 2], G(t) = 1/2 - (2-t)^4/24 + max(1-t, 0)^4/6 (the two pieces
@@ -266,6 +268,40 @@ __device__ __forceinline__ void q_rows2(const T *src, int RW, const T *cv, T &q0
   for (int m = 0; m < N; ++m) {
     q0 += cv[m] * t0[m];
     q1 += cv[m] * t1[m];
+  }
+}
+
+// q_rows2 with a compile-time row pitch: the row offsets become immediate
+// offsets of the LDS reads.
+template <int N, int PITCH, typename T>
+__device__ __forceinline__ void q_rows2c(const T *src, const T *cv, T &q0, T &q1) {
+  T t0[N], t1[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    t0[m] = src[m * PITCH];
+    t1[m] = src[m * PITCH + 64];
+  }
+  q0 = T(0);
+  q1 = T(0);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    q0 += cv[m] * t0[m];
+    q1 += cv[m] * t1[m];
+  }
+}
+
+template <int W, int PITCH, typename T>
+__device__ __forceinline__ void q_n2c(int n, const T *src, const T (&cv)[W], T &q0, T &q1) {
+  q0 = q1 = T(0);
+  switch (n) {
+#define TRTT_Q2C(NN) \
+  case NN:           \
+    if (NN <= W) q_rows2c<(NN <= W ? NN : 1), PITCH, T>(src, cv, q0, q1); \
+    break;
+    TRTT_Q2C(1) TRTT_Q2C(2) TRTT_Q2C(3) TRTT_Q2C(4) TRTT_Q2C(5) TRTT_Q2C(6) TRTT_Q2C(7)
+    TRTT_Q2C(8) TRTT_Q2C(9) TRTT_Q2C(10) TRTT_Q2C(11) TRTT_Q2C(12) TRTT_Q2C(13)
+#undef TRTT_Q2C
+    default: break;
   }
 }
 
@@ -673,9 +709,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
       if (cl.bad) atomicOr(P.err, 1);
     }
-    // this view's columns sit in lanes off .. off + NCOL - 1
-    const int off = multi ? vv * NCOL : 0;
-    const bool live = cl.live && lane >= off && lane < off + NCOL;
+    // this view's columns sit in lanes loff .. loff + NCOL - 1
+    const int loff = multi ? vv * NCOL : 0;
+    const bool live = cl.live && lane >= loff && lane < loff + NCOL;
     // wave-uniform (readfirstlane: kept in scalar registers)
     // the window starts on a multiple of 4 pixels and its rows are a
     // multiple of 4 long (16-byte loads in the reduction)
@@ -945,9 +981,9 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       const T *in = sino + long(view) * npix;
       const double a12 = g.a12;
       const T w0 = T(g.w0), w1 = T(g.w1), wsum = T(g.w0 + g.w1);
-      // this view's columns sit in lanes off .. off + NCOL - 1
-      const int off = vslot * NCOL;
-      const bool live = cl.live && lane >= off && lane < off + NCOL;
+      // this view's columns sit in lanes loff .. loff + NCOL - 1
+      const int loff = vslot * NCOL;
+      const bool live = cl.live && lane >= loff && lane < loff + NCOL;
       int ul, ur, vl, vr;
       block_window(cl, live, red, ul, ur, vl, vr);
       // wave-uniform (readfirstlane: kept in scalar registers)
@@ -958,7 +994,18 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       const bool any = ur > ul && vr > vl;
       const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
       const bool in_lds = any && RW * RH <= lds_cap;
-      if (in_lds) {
+      // Window rows at a fixed pitch of kPitch where they fit (the Q rows
+      // then reach the window rows through immediate offsets); thread c
+      // stages column c of every row.
+      const bool fixed = in_lds && RW <= kPitch && kPitch * RH <= lds_cap;
+      const int pitch = fixed ? kPitch : RW;
+      if (fixed) {
+        const int cI = int(threadIdx.x);
+        if (cI < RW) {
+          long o = long(vl) * P.nu + (ul + cI);
+          for (int r = 0; r < RH; ++r, o += P.nu) region[r * kPitch + cI] = in[o];
+        }
+      } else if (in_lds) {
         // flat over the window with the source offset advanced
         // incrementally (one division per view, not per element)
         const int sr = kThreads / RW, sc = kThreads - sr * RW;
@@ -977,13 +1024,13 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       }
       __syncthreads();
       if (any) {
-        const unsigned long long live_mask = __ballot(live) >> off;
+        const unsigned long long live_mask = __ballot(live) >> loff;
 #pragma unroll 1
         for (int cc = 0; cc < NCOL; ++cc) {
           if (!((live_mask >> cc) & 1ull)) continue;
           Col<T> c;
           int lo, hi, np, i, j;
-          bcast_col(cl, off + cc, c, lo, hi, np, i, j);
+          bcast_col(cl, loff + cc, c, lo, hi, np, i, j);
           np = min(np, W);
           const int span = hi - lo;
           const int npv = c.vr - c.vl;
@@ -994,7 +1041,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 #pragma unroll
           for (int q = 0; q < VPL; ++q) contrib[q] = T(0);
           if (use_q) {
-            const T *src0 = region + (c.vl - vl) * RW + (lo - ul);
+            const T *src0 = region + (c.vl - vl) * pitch + (lo - ul);
             if (span <= 128 * VPL) {
               // 128 pixels per batch of reads; lanes past the span read other
               // entries of the LDS region (index < lds_cap + 128 VPL) and
@@ -1004,14 +1051,17 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
                 const int b = 128 * h;
                 if (h == 0 || span > b) {
                   T q0, q1;
-                  q_n2<W>(npv, src0 + b + lane, RW, cv, q0, q1);
+                  if (fixed)
+                    q_n2c<W, kPitch>(npv, src0 + b + lane, cv, q0, q1);
+                  else
+                    q_n2<W>(npv, src0 + b + lane, RW, cv, q0, q1);
                   if (b + lane < span + W) qrow[b + lane] = b + lane < span ? q0 : T(0);
                   if (b + lane + 64 < span + W)
                     qrow[b + lane + 64] = b + lane + 64 < span ? q1 : T(0);
                 }
               }
             } else {
-              for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, RW, cv);
+              for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, pitch, cv);
             }
             if (lane < W) qrow[span + lane] = T(0);
             __builtin_amdgcn_wave_barrier();
