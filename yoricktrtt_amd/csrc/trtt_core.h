@@ -72,6 +72,16 @@ struct DDConst {
   long nx, ny, nz, nu, nv;
 };
 
+// Rows of the 16-wide pixel tiles of the general 2-D footprint kernels
+// (16 x kTile2DRows pixels per 256-thread workgroup, kTile2DRows / 16 per
+// thread).  Measured on a 512^2 fan plan of 720 views (FP32 cubic, forward /
+// backward): 16 rows 6.51 / 2.24 ms, 32 rows 5.88 / 1.97, 64 rows 5.58 /
+// 1.85, 128 rows 5.60 / 2.49, 256 rows 5.94 / 4.56.
+#ifndef TRTT_TILE2D_Y
+#define TRTT_TILE2D_Y 64
+#endif
+constexpr int kTile2DRows = TRTT_TILE2D_Y;
+
 // Launch shape of the axial kernels of a plan (trtt_hip_plan_set_tuning).
 constexpr int kNumFwdTiles = 3;  // 8x4, 4x4, 8x8 columns x 64 k
 constexpr int kNumBwdTiles = 6;  // 8x8, 8x4, 4x4 columns x 64 k; 8x4, 4x4, 8x8 x 128 k

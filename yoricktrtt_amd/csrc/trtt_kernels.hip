@@ -228,12 +228,15 @@ __device__ __forceinline__ void atomic_add(T *p, T v) {
   unsafeAtomicAdd(p, v);
 }
 
-// Tile of TX x TY x TZ voxels; each of the 256 threads owns NPT voxels
-// stacked along k.
+// Tile of TX x TY x TZ voxels; each of the 256 threads owns NPT voxels,
+// stacked along k, or along j for single-slice tiles larger than the
+// workgroup (the per-view window set-up then serves several pixels).
 template <int TX, int TY, int TZ>
 struct Tile {
   static constexpr int NPT = (TX * TY * TZ) / kThreads;
-  static constexpr int KSTEP = kThreads / (TX * TY);
+  static constexpr bool ALONG_J = TZ == 1 && TX * TY > kThreads;
+  static constexpr int KSTEP = ALONG_J ? 0 : kThreads / (TX * TY);
+  static constexpr int JSTEP = ALONG_J ? kThreads / TX : 0;
   int i, j, k0;
   __device__ __forceinline__ void init(const DevPlan &P) {
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
@@ -244,9 +247,17 @@ struct Tile {
     const int bz = b / nty;
     const int lid = threadIdx.x;
     i = bx * TX + lid % TX;
-    j = by * TY + (lid / TX) % TY;
-    k0 = bz * TZ + lid / (TX * TY);
+    if (ALONG_J) {
+      j = by * TY + lid / TX;
+      k0 = 0;
+    } else {
+      j = by * TY + (lid / TX) % TY;
+      k0 = bz * TZ + lid / (TX * TY);
+    }
   }
+  // coordinates of the thread's voxel q
+  __device__ __forceinline__ int vj(int q) const { return ALONG_J ? j + q * JSTEP : j; }
+  __device__ __forceinline__ int vk(int q) const { return ALONG_J ? k0 : k0 + q * KSTEP; }
 };
 
 // ---------------------------------------------------------------------------
@@ -276,16 +287,16 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     int ul = INT_MAX, vl = INT_MAX, ur = INT_MIN, vr = INT_MIN;
 #pragma unroll
     for (int q = 0; q < TL::NPT; ++q) {
-      const int k = tile.k0 + q * TL::KSTEP;
-      act[q] = tile.i < P.nx && tile.j < P.ny && k < P.nz;
+      const int k = tile.vk(q), jq = tile.vj(q);
+      act[q] = tile.i < P.nx && jq < P.ny && k < P.nz;
       val[q] = T(0);
       if (act[q]) {
-        if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, tile.j, k, f[q])) {
+        if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, jq, k, f[q])) {
           atomicOr(P.err, 1);
           act[q] = false;
           continue;
         }
-        const long idx = (long(k) * P.ny + tile.j) * P.nx + tile.i;
+        const long idx = (long(k) * P.ny + jq) * P.nx + tile.i;
         T x = T(g.w0) * vol[long(g.t0) * P.vox_per_phase + idx];
         if (g.w1 != 0.0) x += T(g.w1) * vol[long(g.t1) * P.vox_per_phase + idx];
         val[q] = x * T(f[q].rho0);
@@ -387,10 +398,10 @@ __global__ __launch_bounds__(kThreads) void fp_backward(DevPlan P, const T *__re
       int ul = INT_MAX, vl = INT_MAX, ur = INT_MIN, vr = INT_MIN;
 #pragma unroll
       for (int q = 0; q < TL::NPT; ++q) {
-        const int k = tile.k0 + q * TL::KSTEP;
-        act[q] = tile.i < P.nx && tile.j < P.ny && k < P.nz;
+        const int k = tile.vk(q), jq = tile.vj(q);
+        act[q] = tile.i < P.nx && jq < P.ny && k < P.nz;
         if (act[q]) {
-          if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, tile.j, k, f[q])) {
+          if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, jq, k, f[q])) {
             atomicOr(P.err, 1);
             act[q] = false;
             continue;
@@ -464,9 +475,9 @@ __global__ __launch_bounds__(kThreads) void fp_backward(DevPlan P, const T *__re
     // flush this segment's accumulators (each voxel is owned by one thread)
 #pragma unroll
     for (int q = 0; q < TL::NPT; ++q) {
-      const int k = tile.k0 + q * TL::KSTEP;
-      if (!(tile.i < P.nx && tile.j < P.ny && k < P.nz)) continue;
-      const long idx = (long(k) * P.ny + tile.j) * P.nx + tile.i;
+      const int k = tile.vk(q), jq = tile.vj(q);
+      if (!(tile.i < P.nx && jq < P.ny && k < P.nz)) continue;
+      const long idx = (long(k) * P.ny + jq) * P.nx + tile.i;
       T *d0 = vol + long(seg.t0) * P.vox_per_phase + idx;
       if (atomic_out) {
         if (acc0[q] != T(0)) atomic_add(d0, acc0[q]);
@@ -895,7 +906,7 @@ template <typename T, int KIND, bool TABLE, bool QUIRK>
 hipError_t launch_fp(const DevPlan &P, int job, const void *in, void *out, const int *seg_begin,
                      int nrows, int views_per_block, int atomic_out, hipStream_t st) {
   constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
-  constexpr int TX = IS2D ? 16 : 8, TY = IS2D ? 16 : 8, TZ = IS2D ? 1 : 8;
+  constexpr int TX = IS2D ? 16 : 8, TY = IS2D ? kTile2DRows : 8, TZ = IS2D ? 1 : 8;
   const int lds_bytes = 32768;
   const int lds_cap = lds_bytes / int(sizeof(T));
   const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY, ntz = (P.nz + TZ - 1) / TZ;
