@@ -317,6 +317,43 @@ def test_batched_plan_2d_vs_oracle(lib, oracle, kind, model):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("kind", ["fan", "parallel2d"])
+@pytest.mark.parametrize("pitch", [0.3, 0.05, 0.01])
+def test_batched_plan_2d_detector_pitch(lib, oracle, kind, pitch, prec):
+    """Detector cells much finer than the pixels: footprints outgrow a
+    thread's 16-cell strip of the FP32 forward (pitch 0.3 on some views,
+    0.05 on all) and a tile's window outgrows the workgroup's 2048-cell LDS
+    window (pitch 0.01), so the forward also takes its per-cell LDS atomic
+    and global atomic paths."""
+    from yoricktrtt_amd.geometry import fan_geometry_2d, parallel_geometry_2d
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("batched2d pitch", kind, pitch, prec)
+    nx, ny, V = 70, 64, 12
+    nv = int(math.ceil(200.0 / pitch))
+    th = np.arange(V) * (2 * math.pi / V) + 0.2
+    if kind == "fan":
+        geo = fan_geometry_2d(nx, ny, 1.0, nv, pitch, th, 200.0, 300.0)
+    else:
+        geo = parallel_geometry_2d(nx, ny, 1.0, nv, pitch, th)
+    rng = np.random.default_rng(21)
+    img = rng.uniform(size=(ny, nx))
+    sino = rng.uniform(size=(V, nv))
+    dt = torch.float32 if prec == "f32" else torch.float64
+    P = Projector(geo, model="cubic", precision=prec)
+    fwd = P.forward(_dev(img, dt)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, dt)).double().cpu().numpy()
+    rb = np.zeros((ny, nx))
+    for v in range(V):
+        rf = oracle.sb2d(np.zeros(nv), img, nx, ny, 1.0, nv, pitch, geo.xos[v], geo.xsd[v],
+                         kind == "fan", 0)
+        assert rel_err(fwd[v], rf) < TOL[prec], v
+        oracle.sb2d(rb, sino[v], nx, ny, 1.0, nv, pitch, geo.xos[v], geo.xsd[v], kind == "fan", 1)
+    assert rel_err(bwd, rb) < TOL[prec]
+    P.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_adjoint_identity(lib, prec):
     from yoricktrtt_amd.geometry import cone_geometry
     from yoricktrtt_amd.projectors import Projector

@@ -255,10 +255,12 @@ void axial_analysis(trtt_hip_plan *p) {
 // Number of workgroups a launch should offer to fill 256 CUs several times.
 constexpr long kTargetBlocks = 4096;
 
+// Tiles of the general kernels (2-D: 1024 pixels, 16 x kTile2DRows in the
+// backward, 32 x 32 in the forward).
 int tiles_of(const trtt_hip_plan *p) {
   const bool is2d = p->kind == trtt::PB2D || p->kind == trtt::FB2D || p->kind == trtt::LF2D;
   const long tx = is2d ? 16 : 8, tz = is2d ? 1 : 8;
-  const long ty = p->kind == trtt::LF2D ? 16 : is2d ? trtt::kTile2DRows : 8;
+  const long ty = is2d ? trtt::kTile2DRows : 8;
   return int(((p->nx + tx - 1) / tx) * ((p->ny + ty - 1) / ty) * ((p->nz + tz - 1) / tz));
 }
 

@@ -81,6 +81,12 @@ struct DDConst {
 #define TRTT_TILE2D_Y 64
 #endif
 constexpr int kTile2DRows = TRTT_TILE2D_Y;
+// Side of the square pixel tiles of the general 2-D forward (kTile2DSquare^2
+// / 256 pixels per thread, laid out per view across the rays).
+#ifndef TRTT_TILE2D_SQUARE
+#define TRTT_TILE2D_SQUARE 32
+#endif
+constexpr int kTile2DSquare = TRTT_TILE2D_SQUARE;
 
 // Launch shape of the axial kernels of a plan (trtt_hip_plan_set_tuning).
 constexpr int kNumFwdTiles = 3;  // 8x4, 4x4, 8x8 columns x 64 k

@@ -238,6 +238,7 @@ struct Tile {
   static constexpr int KSTEP = ALONG_J ? 0 : kThreads / (TX * TY);
   static constexpr int JSTEP = ALONG_J ? kThreads / TX : 0;
   int i, j, k0;
+  int i0, j0;  // tile origin
   __device__ __forceinline__ void init(const DevPlan &P) {
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
     int b = blockIdx.x;
@@ -246,6 +247,8 @@ struct Tile {
     const int by = b % nty;
     const int bz = b / nty;
     const int lid = threadIdx.x;
+    i0 = bx * TX;
+    j0 = by * TY;
     i = bx * TX + lid % TX;
     if (ALONG_J) {
       j = by * TY + lid / TX;
@@ -258,7 +261,31 @@ struct Tile {
   // coordinates of the thread's voxel q
   __device__ __forceinline__ int vj(int q) const { return ALONG_J ? j + q * JSTEP : j; }
   __device__ __forceinline__ int vk(int q) const { return ALONG_J ? k0 : k0 + q * KSTEP; }
+  // Square single-slice tiles, laid out per view: pixel q of the thread
+  // sits at offset lid % TX across the rays and (lid / TX) NPT + q along
+  // them; the across axis is i, or j when flip is set.
+  __device__ __forceinline__ int ai(int q, bool flip) const {
+    return flip ? i0 + int(threadIdx.x / TX) * NPT + q : i;
+  }
+  __device__ __forceinline__ int aj(int q, bool flip) const {
+    return flip ? j0 + int(threadIdx.x % TX) : j0 + int(threadIdx.x / TX) * NPT + q;
+  }
 };
+
+// True when the detector coordinate of a 2-D view varies faster along the
+// image's j axis than along i near pixel (ic, jc) (the derivatives of the
+// footprint centre of voxel_footprint, up to a common factor).
+template <int KIND>
+__device__ __forceinline__ bool detector_along_j(const ViewConst &g, double ic, double jc) {
+  double gi = g.nvx, gj = g.nvy;
+  if (KIND == FB2D) {
+    const double num = (g.nvc + g.nvy * jc) + g.nvx * ic;
+    const double den = (g.dc + g.dy * jc) + g.dx * ic;
+    gi = g.nvx * den - num * g.dx;
+    gj = g.nvy * den - num * g.dy;
+  }
+  return fabs(gj) > fabs(gi);
+}
 
 // ---------------------------------------------------------------------------
 // Forward: sino[view] += A_view (w0 x[t0] + w1 x[t1])
@@ -269,7 +296,16 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
                                                        int lds_cap) {
   constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
   using TL = Tile<TX, TY, TZ>;
-  T *reg = reinterpret_cast<T *>(g_smem);
+  // The LDS atomic adds into the window bound this kernel.  The window
+  // holds FP64 sums whatever T is: on CDNA4 an LDS atomic add of FP64 runs
+  // several times faster than one of FP32 (configuration-2 sizes, FP32
+  // plan: 5.2 ms with an FP32 window, 1.9 ms with an FP64 one).  Square 2-D
+  // tiles choose per view whether a wave's 64 lanes run along i or along j:
+  // across the rays, so that the lanes' footprints start on different
+  // detector cells (adds to one address serialise).
+  constexpr bool FLIP = IS2D && TL::ALONG_J && TX == TY;
+  using W = double;
+  W *reg = reinterpret_cast<W *>(g_smem);
   __shared__ int red[4 * (kThreads / 64)];
   TL tile;
   tile.init(P);
@@ -280,6 +316,8 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     if (view >= P.nviews) break;
     const ViewConst &g = P.views[view];
     T *out = sino + long(view) * npix;
+    const bool flip =
+        FLIP && detector_along_j<KIND>(g, tile.i0 + 0.5 * (TX - 1), tile.j0 + 0.5 * (TY - 1));
 
     Foot f[TL::NPT];
     T val[TL::NPT];
@@ -287,16 +325,17 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     int ul = INT_MAX, vl = INT_MAX, ur = INT_MIN, vr = INT_MIN;
 #pragma unroll
     for (int q = 0; q < TL::NPT; ++q) {
-      const int k = tile.vk(q), jq = tile.vj(q);
-      act[q] = tile.i < P.nx && jq < P.ny && k < P.nz;
+      const int k = tile.vk(q);
+      const int iq = FLIP ? tile.ai(q, flip) : tile.i, jq = FLIP ? tile.aj(q, flip) : tile.vj(q);
+      act[q] = iq < P.nx && jq < P.ny && k < P.nz;
       val[q] = T(0);
       if (act[q]) {
-        if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, tile.i, jq, k, f[q])) {
+        if (!voxel_footprint<KIND, TABLE, QUIRK>(P, g, iq, jq, k, f[q])) {
           atomicOr(P.err, 1);
           act[q] = false;
           continue;
         }
-        const long idx = (long(k) * P.ny + jq) * P.nx + tile.i;
+        const long idx = (long(k) * P.ny + jq) * P.nx + iq;
         T x = T(g.w0) * vol[long(g.t0) * P.vox_per_phase + idx];
         if (g.w1 != 0.0) x += T(g.w1) * vol[long(g.t1) * P.vox_per_phase + idx];
         val[q] = x * T(f[q].rho0);
@@ -314,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     const int RW = any ? ur - ul : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && long(RW) * RH <= lds_cap;
     if (in_lds) {
-      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) reg[idx] = T(0);
+      for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) reg[idx] = W(0);
       __syncthreads();
     }
 #pragma unroll
@@ -338,10 +377,10 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
         for (int mm = 0; mm < npv; ++mm) {
           const T c = av.next(mm) * val[q];
           if (in_lds) {
-            T *row = reg + (F.vl + mm - vl) * RW + (F.ul + l0 - ul);
+            W *row = reg + (F.vl + mm - vl) * RW + (F.ul + l0 - ul);
 #pragma unroll
             for (int l = 0; l < kMaxW; ++l)
-              if (l0 + l < npu) atomicAdd(row + l, c * wu[l]);
+              if (l0 + l < npu) atomicAdd(row + l, W(c * wu[l]));
           } else {
             T *row = out + long(F.vl + mm) * P.nu + (F.ul + l0);
 #pragma unroll
@@ -354,10 +393,10 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     if (in_lds) {
       __syncthreads();
       for (int idx = threadIdx.x; idx < RW * RH; idx += kThreads) {
-        const T v = reg[idx];
-        if (v != T(0)) {
+        const W v = reg[idx];
+        if (v != W(0)) {
           const int r = idx / RW, c = idx - r * RW;
-          atomic_add(out + long(vl + r) * P.nu + (ul + c), v);
+          atomic_add(out + long(vl + r) * P.nu + (ul + c), T(v));
         }
       }
     }
@@ -548,46 +587,70 @@ __device__ __forceinline__ double lf_gamma(const LFFoot &f, int k, double size) 
          lf_ramp(t[3] - (s2 <= t[3] ? s2 : t[3]), t[3] - (s1 >= t[2] ? s1 : t[2]), t[3] - t[2]);
 }
 
+// True when the detector coordinate of a Long & Fessler view varies faster
+// along the image's y axis (i2) than along x (i1) at (x, y): the
+// derivatives of lf_footprint's corner projection, up to a common factor.
+__device__ __forceinline__ bool lf_detector_along_y(const ViewConst &g, double R, double x,
+                                                    double y) {
+  const double cb = g.lf_cb, sb = g.lf_sb;
+  const double num = y * sb + x * cb, den = R - (y * cb - x * sb);
+  return fabs(sb * den + num * cb) > fabs(cb * den - num * sb);
+}
+
+// Square tiles laid out per view and an FP64 LDS window, as in
+// fp_forward's 2-D path.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void lf_forward(DevPlan P, const T *__restrict__ img,
                                                        T *__restrict__ sino, int views_per_block,
                                                        int lds_cap) {
-  T *reg = reinterpret_cast<T *>(g_smem);
+  constexpr int S = kTile2DSquare;
+  using TL = Tile<S, S, 1>;
+  double *reg = reinterpret_cast<double *>(g_smem);
   __shared__ int red[4 * (kThreads / 64)];
-  Tile<16, 16, 1> tile;
+  TL tile;
   tile.init(P);
-  const bool act0 = tile.i < P.nx && tile.j < P.ny;
   for (int vv = 0; vv < views_per_block; ++vv) {
     const int view = blockIdx.y * views_per_block + vv;
     if (view >= P.nviews) break;
     const ViewConst &g = P.views[view];
     T *out = sino + long(view) * P.nv;
-    LFFoot f;
-    T val = T(0);
-    bool act = act0;
+    const double xc = g.lf_x0 + (tile.i0 + 0.5 * (S - 1)) * P.lf_vs;
+    const double yc = g.lf_y0 + (tile.j0 + 0.5 * (S - 1)) * P.lf_vs;
+    const bool flip = lf_detector_along_y(g, P.lf_R, xc, yc);
+    LFFoot f[TL::NPT];
+    T val[TL::NPT];
+    bool act[TL::NPT];
     int lo = INT_MAX, hi = INT_MIN, d0 = 0, d1 = 0;
-    if (act) {
-      lf_footprint(P, g, tile.i, tile.j, f);
-      val = img[long(tile.j) * P.nx + tile.i];
-      act = f.kf <= f.kl;
-      if (act) {
-        lo = f.kf;
-        hi = f.kl + 1;
+#pragma unroll
+    for (int q = 0; q < TL::NPT; ++q) {
+      const int iq = tile.ai(q, flip), jq = tile.aj(q, flip);
+      act[q] = iq < P.nx && jq < P.ny;
+      val[q] = T(0);
+      if (act[q]) {
+        lf_footprint(P, g, iq, jq, f[q]);
+        val[q] = img[long(jq) * P.nx + iq];
+        act[q] = f[q].kf <= f[q].kl;
+        if (act[q]) {
+          lo = min(lo, f[q].kf);
+          hi = max(hi, f[q].kl + 1);
+        }
       }
     }
     block_window(lo, hi, d0, d1, red);
     const int RW = hi > lo ? hi - lo : 0;
     const bool in_lds = RW > 0 && RW <= lds_cap;
     if (in_lds) {
-      for (int idx = threadIdx.x; idx < RW; idx += kThreads) reg[idx] = T(0);
+      for (int idx = threadIdx.x; idx < RW; idx += kThreads) reg[idx] = 0.0;
       __syncthreads();
     }
-    if (act) {
-      const double a = f.amp * double(val);
-      for (int k = f.kf; k <= f.kl; ++k) {
-        const T c = T(lf_gamma(f, k, P.lf_size) * a);
+#pragma unroll
+    for (int q = 0; q < TL::NPT; ++q) {
+      if (!act[q]) continue;
+      const double a = f[q].amp * double(val[q]);
+      for (int k = f[q].kf; k <= f[q].kl; ++k) {
+        const T c = T(lf_gamma(f[q], k, P.lf_size) * a);
         if (in_lds)
-          atomicAdd(reg + (k - lo), c);
+          atomicAdd(reg + (k - lo), double(c));
         else
           atomic_add(out + k, c);
       }
@@ -595,8 +658,8 @@ __global__ __launch_bounds__(kThreads) void lf_forward(DevPlan P, const T *__res
     if (in_lds) {
       __syncthreads();
       for (int idx = threadIdx.x; idx < RW; idx += kThreads) {
-        const T v = reg[idx];
-        if (v != T(0)) atomic_add(out + lo + idx, v);
+        const double v = reg[idx];
+        if (v != 0.0) atomic_add(out + lo + idx, T(v));
       }
     }
     __syncthreads();
@@ -902,6 +965,10 @@ hipError_t launch_dd(bool f64, const DDConst *cs, const DDConst &c0, int nviews,
 // ---------------------------------------------------------------------------
 // Host-side launchers (explicit instantiation table)
 // ---------------------------------------------------------------------------
+#ifndef TRTT_FWD3D_LDS
+#define TRTT_FWD3D_LDS 32768
+#endif
+
 template <typename T, int KIND, bool TABLE, bool QUIRK>
 hipError_t launch_fp(const DevPlan &P, int job, const void *in, void *out, const int *seg_begin,
                      int nrows, int views_per_block, int atomic_out, hipStream_t st) {
@@ -912,10 +979,16 @@ hipError_t launch_fp(const DevPlan &P, int job, const void *in, void *out, const
   const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY, ntz = (P.nz + TZ - 1) / TZ;
   const unsigned ntiles = unsigned(ntx) * nty * ntz;
   if (job == 0) {
-    dim3 grid(ntiles, (P.nviews + views_per_block - 1) / views_per_block);
-    hipLaunchKernelGGL((fp_forward<T, KIND, TABLE, QUIRK, TX, TY, TZ>), grid, dim3(kThreads),
-                       lds_bytes, st, P, static_cast<const T *>(in), static_cast<T *>(out),
-                       views_per_block, lds_cap);
+    // the 2-D forward takes square tiles (lanes laid out per view)
+    // (FP64 window of up to 2048 cells in 2-D, TRTT_FWD3D_LDS bytes in 3-D)
+    constexpr int FX = IS2D ? kTile2DSquare : TX, FY = IS2D ? kTile2DSquare : TY;
+    const unsigned nft = unsigned((P.nx + FX - 1) / FX) * ((P.ny + FY - 1) / FY) * ntz;
+    const int lds_f = IS2D ? 8 * 2048 : TRTT_FWD3D_LDS;
+    const int cap_f = lds_f / 8;
+    dim3 grid(nft, (P.nviews + views_per_block - 1) / views_per_block);
+    hipLaunchKernelGGL((fp_forward<T, KIND, TABLE, QUIRK, FX, FY, TZ>), grid, dim3(kThreads),
+                       lds_f, st, P, static_cast<const T *>(in), static_cast<T *>(out),
+                       views_per_block, cap_f);
   } else {
     dim3 grid(ntiles, nrows);
     hipLaunchKernelGGL((fp_backward<T, KIND, TABLE, QUIRK, TX, TY, TZ>), grid, dim3(kThreads),
@@ -950,17 +1023,19 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
                             int views_per_block, int atomic_out, hipStream_t st) {
   if (kind == LF2D) {
     const int ntx = (P.nx + 15) / 16, nty = (P.ny + 15) / 16;
-    const int lds_bytes = 16384;
     dim3 grid(unsigned(ntx) * nty, (P.nviews + views_per_block - 1) / views_per_block);
     if (job == 0) {
+      // square tiles, an FP64 window of up to 2048 cells
+      constexpr int S = kTile2DSquare, cap = 2048;
+      const dim3 fgrid(unsigned((P.nx + S - 1) / S) * ((P.ny + S - 1) / S), grid.y);
       if (f64)
-        hipLaunchKernelGGL(lf_forward<double>, grid, dim3(kThreads), lds_bytes, st, P,
+        hipLaunchKernelGGL(lf_forward<double>, fgrid, dim3(kThreads), cap * 8, st, P,
                            static_cast<const double *>(in), static_cast<double *>(out),
-                           views_per_block, lds_bytes / 8);
+                           views_per_block, cap);
       else
-        hipLaunchKernelGGL(lf_forward<float>, grid, dim3(kThreads), lds_bytes, st, P,
+        hipLaunchKernelGGL(lf_forward<float>, fgrid, dim3(kThreads), cap * 8, st, P,
                            static_cast<const float *>(in), static_cast<float *>(out),
-                           views_per_block, lds_bytes / 4);
+                           views_per_block, cap);
     } else {
       if (f64)
         hipLaunchKernelGGL(lf_backward<double>, grid, dim3(kThreads), 0, st, P,
