@@ -238,7 +238,7 @@ struct Tile {
   static constexpr int KSTEP = ALONG_J ? 0 : kThreads / (TX * TY);
   static constexpr int JSTEP = ALONG_J ? kThreads / TX : 0;
   int i, j, k0;
-  int i0, j0;  // tile origin
+  int i0, j0, kb;  // tile origin
   __device__ __forceinline__ void init(const DevPlan &P) {
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
     int b = blockIdx.x;
@@ -249,6 +249,7 @@ struct Tile {
     const int lid = threadIdx.x;
     i0 = bx * TX;
     j0 = by * TY;
+    kb = bz * TZ;
     i = bx * TX + lid % TX;
     if (ALONG_J) {
       j = by * TY + lid / TX;
@@ -270,19 +271,35 @@ struct Tile {
   __device__ __forceinline__ int aj(int q, bool flip) const {
     return flip ? j0 + int(threadIdx.x % TX) : j0 + int(threadIdx.x / TX) * NPT + q;
   }
+  // Cubic 3-D tiles, laid out per view: a wave's lanes cover TX offsets
+  // across the rays times TZ slices, and the thread's NPT voxels follow one
+  // another along the rays; the across axis is i, or j when flip is set.
+  static constexpr bool CUBE = !ALONG_J && TX == TY && TX * TZ == 64 && TX * TY * TZ > kThreads;
+  __device__ __forceinline__ void cube(int q, bool flip, int &vi, int &vjj, int &vkk) const {
+    const int a = int(threadIdx.x % TX);
+    const int b = int(threadIdx.x / (TX * TZ)) * NPT + q;
+    vkk = kb + int(threadIdx.x / TX) % TZ;
+    vi = i0 + (flip ? b : a);
+    vjj = j0 + (flip ? a : b);
+  }
 };
 
-// True when the detector coordinate of a 2-D view varies faster along the
-// image's j axis than along i near pixel (ic, jc) (the derivatives of the
-// footprint centre of voxel_footprint, up to a common factor).
+// True when the detector coordinate across the rays (v of a 2-D view, u of
+// a 3-D one) varies faster along the image's j axis than along i near
+// voxel (ic, jc, kc): the derivatives of the footprint centre of
+// voxel_footprint, up to a common factor.
 template <int KIND>
-__device__ __forceinline__ bool detector_along_j(const ViewConst &g, double ic, double jc) {
-  double gi = g.nvx, gj = g.nvy;
-  if (KIND == FB2D) {
-    const double num = (g.nvc + g.nvy * jc) + g.nvx * ic;
-    const double den = (g.dc + g.dy * jc) + g.dx * ic;
-    gi = g.nvx * den - num * g.dx;
-    gj = g.nvy * den - num * g.dy;
+__device__ __forceinline__ bool detector_along_j(const ViewConst &g, double ic, double jc,
+                                                 double kc) {
+  constexpr bool IS2D = (KIND == PB2D || KIND == FB2D);
+  const double nx = IS2D ? g.nvx : g.nux, ny = IS2D ? g.nvy : g.nuy;
+  const double nz = IS2D ? g.nvz : g.nuz, nc = IS2D ? g.nvc : g.nuc;
+  double gi = nx, gj = ny;
+  if (KIND == FB2D || KIND == CB3D) {
+    const double num = ((nz * kc + nc) + ny * jc) + nx * ic;
+    const double den = ((g.dz * kc + g.dc) + g.dy * jc) + g.dx * ic;
+    gi = nx * den - num * g.dx;
+    gj = ny * den - num * g.dy;
   }
   return fabs(gj) > fabs(gi);
 }
@@ -300,10 +317,12 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
   // holds FP64 sums whatever T is: on CDNA4 an LDS atomic add of FP64 runs
   // several times faster than one of FP32 (configuration-2 sizes, FP32
   // plan: 5.2 ms with an FP32 window, 1.9 ms with an FP64 one).  Square 2-D
-  // tiles choose per view whether a wave's 64 lanes run along i or along j:
-  // across the rays, so that the lanes' footprints start on different
-  // detector cells (adds to one address serialise).
+  // tiles and cubic 3-D tiles choose per view whether a wave's lanes run
+  // along i or along j (times k in 3-D): across the rays, so that the lanes'
+  // footprints start on different detector cells (adds to one address
+  // serialise).
   constexpr bool FLIP = IS2D && TL::ALONG_J && TX == TY;
+  constexpr bool CUBE = TL::CUBE && !IS2D;
   using W = double;
   W *reg = reinterpret_cast<W *>(g_smem);
   __shared__ int red[4 * (kThreads / 64)];
@@ -316,8 +335,9 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     if (view >= P.nviews) break;
     const ViewConst &g = P.views[view];
     T *out = sino + long(view) * npix;
-    const bool flip =
-        FLIP && detector_along_j<KIND>(g, tile.i0 + 0.5 * (TX - 1), tile.j0 + 0.5 * (TY - 1));
+    const bool flip = (FLIP || CUBE) &&
+                      detector_along_j<KIND>(g, tile.i0 + 0.5 * (TX - 1), tile.j0 + 0.5 * (TY - 1),
+                                             tile.kb + 0.5 * (TZ - 1));
 
     Foot f[TL::NPT];
     T val[TL::NPT];
@@ -325,8 +345,14 @@ __global__ __launch_bounds__(kThreads) void fp_forward(DevPlan P, const T *__res
     int ul = INT_MAX, vl = INT_MAX, ur = INT_MIN, vr = INT_MIN;
 #pragma unroll
     for (int q = 0; q < TL::NPT; ++q) {
-      const int k = tile.vk(q);
-      const int iq = FLIP ? tile.ai(q, flip) : tile.i, jq = FLIP ? tile.aj(q, flip) : tile.vj(q);
+      int iq, jq, k;
+      if (CUBE) {
+        tile.cube(q, flip, iq, jq, k);
+      } else {
+        k = tile.vk(q);
+        iq = FLIP ? tile.ai(q, flip) : tile.i;
+        jq = FLIP ? tile.aj(q, flip) : tile.vj(q);
+      }
       act[q] = iq < P.nx && jq < P.ny && k < P.nz;
       val[q] = T(0);
       if (act[q]) {
