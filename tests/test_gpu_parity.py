@@ -502,6 +502,44 @@ def test_axial_wide_footprints_and_truncation(lib, oracle):
     P.close()
 
 
+@pytest.mark.parametrize("kind,us", [("parallel", 1.5), ("parallel", 2.5), ("parallel", 3.5),
+                                     ("parallel", 4.5), ("cone", 2.5), ("cone", 5.0)])
+def test_axial_coarse_detector_vs_oracle(lib, oracle, kind, us):
+    """Detector pixels along the rotation axis coarser than the projected
+    slice pitch: voxels of neighbouring slices start their windows on the
+    same pixel, so the forward accumulates its column profiles in 2, 3 or 4
+    interleaved plain passes (parallel pitch 1.5 / 2.5 / 3.5, cone 2.5 / 5)
+    or with LDS atomics (parallel 4.5)."""
+    from yoricktrtt_amd.geometry import cone_geometry, parallel_geometry_3d
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("axial coarse detector", kind, us)
+    nx, ny, nz, nv, V = 24, 22, 64, 48, 5
+    th = np.linspace(0.1, 5.0, V)
+    if kind == "cone":
+        nu = int(math.ceil(64 * 1.8 / us)) + 6
+        geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, us, 1.0, th, 70.0, 120.0)
+    else:
+        nu = int(math.ceil(64 / us)) + 6
+        geo = parallel_geometry_3d(nx, ny, nz, 1.0, nu, nv, us, 1.0, th)
+    P = Projector(geo, model="cubic", precision="f32")
+    assert P.path in (9, 13)
+    rng = np.random.default_rng(int(us * 10))
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, 1.0, geo.xos[v],
+                         geo.xsd[v], kind == "cone", 0)
+        assert rel_err(fwd[v], rf) < 1e-5, v
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, 1.0, geo.xos[v], geo.xsd[v],
+                    kind == "cone", 1)
+    assert rel_err(bwd, rb) < 1e-5
+    P.close()
+
+
 @pytest.mark.parametrize("nt", [1, 3])
 def test_axial_backward_tiles_vs_oracle(lib, oracle, nt):
     """Every backward launch shape, including the tiles of 128 slices (two

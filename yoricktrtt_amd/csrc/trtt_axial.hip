@@ -390,17 +390,33 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
 // Forward profile steps of one voxel whose window has at most NP pixels
 // (pp: the profile at the window's first pixel, unclamped: W margin entries
 // precede the profile, spare entries follow it, and neither is read).  The
-// pixels past the footprint get zero weights.  Where the host has shown that
-// consecutive voxels start their windows on strictly monotone pixels (pl,
-// wave-uniform), plain read-modify-write steps; otherwise LDS atomics.
+// pixels past the footprint get zero weights.  passes (wave-uniform) comes
+// from the host: 1 where it has shown that consecutive voxels start their
+// windows on distinct pixels (plain read-modify-write steps, the lanes hit
+// distinct entries at each step and LDS serves one wave's steps in order);
+// m > 1 where voxels m slices apart do, so the lanes of one residue of k
+// mod m run the plain steps together, one residue after the other; 0:
+// LDS atomics.
 template <int NP, typename T>
-__device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, bool pl) {
+__device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, int passes) {
   T w[NP];
   uweights<NP>(v, w);
-  if (pl) {
+  if (passes == 1) {
 #pragma unroll
     for (int l = 0; l < NP; ++l) {
       pp[l] += w[l] * val;
+      __builtin_amdgcn_wave_barrier();
+    }
+  } else if (passes > 1) {
+    const int r = int(threadIdx.x & 63) % passes;
+    for (int ps = 0; ps < passes; ++ps) {
+      if (r == ps) {
+#pragma unroll
+        for (int l = 0; l < NP; ++l) {
+          pp[l] += w[l] * val;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
       __builtin_amdgcn_wave_barrier();
     }
   } else {
@@ -424,7 +440,7 @@ __device__ __forceinline__ T gather_steps(const Vox<T> &v, const T *qq) {
 
 // Dispatch on the column's wave-uniform pixel bound np (1..W).
 template <int W, typename T>
-__device__ __forceinline__ void profile_n(int np, const Vox<T> &v, T *pp, T val, bool pl) {
+__device__ __forceinline__ void profile_n(int np, const Vox<T> &v, T *pp, T val, int pl) {
   switch (np) {
 #define TRTT_PS(NN) \
   case NN:          \
@@ -625,9 +641,9 @@ __device__ __forceinline__ void block_window(const ColLane<T> &cl, bool live, in
 // Forward.  One wave per workgroup: the wave owns a tile of CX x CY columns
 // x 64 k and a private detector window in LDS, so no barrier or atomic is
 // needed between waves.  Per column the lanes accumulate the axial profile
-// P[u] = sum_k x_k rho_k w_k(u) in LDS (plain read-modify-write when the
-// host has shown that the lanes hit distinct pixels at each step, LDS atomics
-// otherwise), then add the outer product P x cv to the window with the lanes
+// P[u] = sum_k x_k rho_k w_k(u) in LDS (plain read-modify-write steps in
+// `plain` interleaved passes when the host has shown that the lanes of a
+// pass hit distinct pixels at each step, LDS atomics when plain is 0), then add the outer product P x cv to the window with the lanes
 // on distinct window columns.  The window goes to a scratch slot; fwd_reduce
 // sums the slots covering each detector pixel.
 // ---------------------------------------------------------------------------
@@ -779,7 +795,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           // voxels that need the direct path take it in a separate block.
           const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
           profile_n<W>(np, v, fast ? prof + (v.ul - lo) : prof - W, fast ? val : T(0),
-                       plain != 0);
+                       plain);
           if (__ballot(act && !fast)) {
             T dummy = T(0);
             if (act && !fast) slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);

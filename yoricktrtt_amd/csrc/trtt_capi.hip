@@ -244,12 +244,19 @@ void axial_analysis(trtt_hip_plan *p) {
   // backward tiles of 128 k: the profile over 128 slices must fit too
   const double pmax2 = std::ceil(127.0 * duc_max + 2.0 * hs0 * hu_max) + 8.0;
   p->axial_pmax2 = pmax2 <= 256.0 ? int(pmax2) : 0;
-  // consecutive voxels of a column start their windows on distinct pixels
-  p->axial_plain = duc_min - 2.0 * dhs_rate > 1.01;
+  // voxels m slices apart start their windows on distinct pixels once
+  // m (duc_min - 2 dhs_rate) > 1.01: the forward then accumulates its
+  // profiles in m plain passes (1 for consecutive voxels), or with LDS
+  // atomics beyond kMaxProfilePasses
+  constexpr int kMaxProfilePasses = 4;
+  const double rate = duc_min - 2.0 * dhs_rate;
+  p->axial_plain = rate > 1.01 ? 1
+                   : rate > 1.01 / kMaxProfilePasses ? int(std::floor(1.01 / rate)) + 1
+                                                     : 0;
   // TRTT_HIP_PROFILE_ATOMIC=1 accumulates the axial profiles with LDS
   // atomics even where plain accumulation is safe (same result)
   if (const char *e = std::getenv("TRTT_HIP_PROFILE_ATOMIC"))
-    if (e[0] == '1') p->axial_plain = false;
+    if (e[0] == '1') p->axial_plain = 0;
 }
 
 // Number of workgroups a launch should offer to fill 256 CUs several times.
@@ -619,7 +626,7 @@ hipError_t launch_plan(trtt_hip_plan *p, int job, const void *in, void *out, hip
     return trtt::launch_axial(p->kind == trtt::CB3D, p->axial_w, f64, p->dev, job, in, out,
                               p->d_seg_begin, nrows, int(vpb), nrows > 1 ? 1 : 0, p->axial_pmax,
                               p->axial_pmax2, p->d_scratch, p->d_hdr, p->axial_chunk, p->d_volk,
-                              p->axial_plain ? 1 : 0, p->tune, ev, &p->overlap, p->axial_nbuf,
+                              p->axial_plain, p->tune, ev, &p->overlap, p->axial_nbuf,
                               p->two_phase_segs ? 1 : 0, st);
   }
   if (job == 0)

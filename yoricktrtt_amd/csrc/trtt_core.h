@@ -140,7 +140,7 @@ struct trtt_hip_plan {
   int axial_w = 0;
   int axial_pmax = 0;
   int axial_pmax2 = 0;       // column profile length over 128 k (0: over 256)
-  bool axial_plain = false;   // profile accumulation without LDS atomics
+  int axial_plain = 0;       // profile passes without LDS atomics (0: atomics)
   int axial_chunk = 1;      // views per forward scratch round
   void *d_scratch = nullptr;  // forward windows, one per (view, tile)
   void *d_hdr = nullptr;      // int4 (ul, vl, width, height) per window
