@@ -14,6 +14,7 @@
 //      row loads and gathers from LDS.
 // A window larger than the LDS budget falls back to direct global atomics /
 // loads for that (tile, view).
+#include <algorithm>
 #include <climits>
 
 #include "trtt_core.h"
@@ -991,6 +992,9 @@ hipError_t launch_dd(bool f64, const DDConst *cs, const DDConst &c0, int nviews,
 // ---------------------------------------------------------------------------
 // Host-side launchers (explicit instantiation table)
 // ---------------------------------------------------------------------------
+#ifndef TRTT_LF_BWD_BLOCKS
+#define TRTT_LF_BWD_BLOCKS 8192
+#endif
 #ifndef TRTT_FWD3D_LDS
 #define TRTT_FWD3D_LDS 32768
 #endif
@@ -1063,14 +1067,23 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
                            static_cast<const float *>(in), static_cast<float *>(out),
                            views_per_block, cap);
     } else {
+      // the views split over grid rows (atomic adds into the image) until
+      // the grid offers TRTT_LF_BWD_BLOCKS workgroups: one row of 16x16
+      // tiles alone leaves one wave per SIMD on a 512^2 image
+      const long tiles = long(ntx) * nty;
+      const long span = std::max(1, std::min(views_per_block, P.nviews));
+      const long want = (long(TRTT_LF_BWD_BLOCKS) + tiles - 1) / tiles;
+      const int vpb = int(std::max(1L, std::min(span, (P.nviews + want - 1) / want)));
+      const dim3 bgrid(unsigned(tiles), unsigned((P.nviews + vpb - 1) / vpb));
+      const int atomic = (atomic_out || bgrid.y > 1) ? 1 : 0;
       if (f64)
-        hipLaunchKernelGGL(lf_backward<double>, grid, dim3(kThreads), 0, st, P,
-                           static_cast<const double *>(in), static_cast<double *>(out),
-                           views_per_block, atomic_out);
+        hipLaunchKernelGGL(lf_backward<double>, bgrid, dim3(kThreads), 0, st, P,
+                           static_cast<const double *>(in), static_cast<double *>(out), vpb,
+                           atomic);
       else
-        hipLaunchKernelGGL(lf_backward<float>, grid, dim3(kThreads), 0, st, P,
-                           static_cast<const float *>(in), static_cast<float *>(out),
-                           views_per_block, atomic_out);
+        hipLaunchKernelGGL(lf_backward<float>, bgrid, dim3(kThreads), 0, st, P,
+                           static_cast<const float *>(in), static_cast<float *>(out), vpb,
+                           atomic);
     }
     return hipGetLastError();
   }
