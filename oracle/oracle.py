@@ -66,6 +66,13 @@ class Oracle:
         lib.orc_sb3d_views_fwd.argtypes = [dp, dp, L, L, L, D, I, I, dp, L, D, D, L, L, D, D, dp, dp,
                                            L, I]
         lib.orc_sb3d_views_fwd.restype = I
+        lib.orc_sb3d_views_bwd.argtypes = [dp, dp, L, L, L, D, I, I, dp, L, D, D, L, L, D, D, dp, dp,
+                                           L, I, L, L]
+        lib.orc_sb3d_views_bwd.restype = I
+        lib.orc_sb2d_views.argtypes = [dp, dp, L, L, D, I, I, dp, L, D, D, L, D, dp, dp, L, I, I]
+        lib.orc_sb2d_views.restype = I
+        lib.orc_lf2d_views.argtypes = [C.POINTER(LF2DParams), dp, dp, dp, dp, L, I]
+        lib.orc_lf2d_views.restype = I
         lib.orc_sb3d_count_pairs.argtypes = [L, L, L, D, I, L, L, D, D, dp, dp, I]
         lib.orc_sb3d_count_pairs.restype = D
         lib.orc_lf2d.argtypes = [C.POINTER(LF2DParams), D, D, dp, dp, I, I]
@@ -121,6 +128,51 @@ class Oracle:
         if r != 0:
             raise RuntimeError("oracle projector failure")
         return sino
+
+    def views_bwd3d(self, sino, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, cone, table=None, deg=3,
+                    k0=0, k1=None):
+        """Back projection of every view, restricted to the slices
+        k0 <= k < k1 (OpenMP over views): returns the slab [k1-k0][ny][nx]."""
+        kind, cum, m, step, c0 = self._fp(table, FP_TABLE3D)
+        k1 = nz if k1 is None else k1
+        nview = xos.shape[0]
+        sino = _f64(sino)
+        assert sino.size == nview * nv * nu
+        vol = np.zeros((k1 - k0, ny, nx))
+        r = self.lib.orc_sb3d_views_bwd(_p(vol), _p(sino), nx, ny, nz, s, deg, kind, _p(cum), m,
+                                        step, c0, nu, nv, us, vs, _p(_f64(xos)), _p(_f64(xsd)),
+                                        nview, int(cone), int(k0), int(k1))
+        if r != 0:
+            raise RuntimeError("oracle projector failure")
+        return vol
+
+    def views2d(self, inp, nx, ny, s, nv, vs, xos, xsd, fan, job, table=None, deg=3):
+        """2-D projector over every view (OpenMP over views): job 0 returns
+        the sinogram [view][nv] of image ``inp``; job 1 the back projection
+        of sinogram ``inp``."""
+        kind, cum, m, step, c0 = self._fp(table, FP_TABLE2D)
+        nview = xos.shape[0]
+        inp = _f64(inp)
+        out = np.zeros((nview, nv)) if job == 0 else np.zeros((ny, nx))
+        r = self.lib.orc_sb2d_views(_p(out), _p(inp), nx, ny, s, deg, kind, _p(cum), m, step, c0,
+                                    nv, vs, _p(_f64(xos)), _p(_f64(xsd)), nview, int(fan),
+                                    int(job))
+        if r != 0:
+            raise RuntimeError("oracle projector failure")
+        return out
+
+    def lf2d_views(self, params: LF2DParams, beta, offset, src, job):
+        """Long & Fessler projector over every view: job 0 returns the
+        sinogram [view][nd] of image ``src``; job 1 the back projection."""
+        beta, offset = _f64(beta), _f64(offset)
+        nview, nd = beta.size, params.detectorDimension
+        out = (np.zeros((nview, nd)) if job == 0
+               else np.zeros((params.objectDimension2, params.objectDimension1)))
+        r = self.lib.orc_lf2d_views(C.byref(params), _p(beta), _p(offset), _p(_f64(src)), _p(out),
+                                    nview, int(job))
+        if r != 0:
+            raise RuntimeError("oracle projector failure")
+        return out
 
     def count_pairs(self, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, cone, deg=3) -> float:
         return self.lib.orc_sb3d_count_pairs(nx, ny, nz, s, deg, nu, nv, us, vs, _p(_f64(xos)),

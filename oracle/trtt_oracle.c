@@ -259,10 +259,29 @@ static void view3d_setup(view3d_t *g, const double *xos, const double *xsd,
  * job==0: out[nv][nu] += A in[nz][ny][nx]; job!=0: out[vol] += A' in[det].
  * Voxel geometry is accumulated in the reference's incremental order
  * (k, then j, then i; trtt_3D_projectors.c:1234-1261). */
+static int sb3d_slab(double *out, const double *in, long nx, long ny, long nz,
+                     double s, int s_deg, const orc_fp_t *fp, long nu, long nv,
+                     double u_scl, double v_scl, const double *xos,
+                     const double *xsd, int cone, int job, long k0, long k1);
+
 int orc_sb3d(double *out, const double *in, long nx, long ny, long nz,
              double s, int s_deg, const orc_fp_t *fp, long nu, long nv,
              double u_scl, double v_scl, const double *xos, const double *xsd,
              int cone, int job)
+{
+    return sb3d_slab(out, in, nx, ny, nz, s, s_deg, fp, nu, nv, u_scl, v_scl,
+                     xos, xsd, cone, job, 0, nz);
+}
+
+/* The projector restricted to the slices k0 <= k < k1.  The per-voxel
+ * arithmetic does not depend on the restriction (each slice starts its
+ * incremental sums from k).  job != 0 writes the slab only:
+ * out[((k-k0)*ny + j)*nx + i]; job == 0 projects the slab's voxels of the
+ * full volume `in`. */
+static int sb3d_slab(double *out, const double *in, long nx, long ny, long nz,
+                     double s, int s_deg, const orc_fp_t *fp, long nu, long nv,
+                     double u_scl, double v_scl, const double *xos,
+                     const double *xsd, int cone, int job, long k0, long k1)
 {
     view3d_t g;
     long i, j, k, l, m;
@@ -272,8 +291,9 @@ int orc_sb3d(double *out, const double *in, long nx, long ny, long nz,
     double *wv = (double *)malloc(sizeof(double) * (size_t)(nv + 1));
     if (wu == NULL || wv == NULL) { free(wu); free(wv); return ORC_FAIL; }
     view3d_setup(&g, xos, xsd, s, s_deg, u_scl, v_scl);
+    (void)nz;
 
-    for (k = 0; k < nz; ++k) {
+    for (k = k0; k < k1; ++k) {
         double wk1 = g.o3 * k + g.o4, nu1 = g.nuz * k + g.nuc;
         double nv1 = g.nvz * k + g.nvc, de1 = g.dz * k + g.dc;
         double pv1 = g.vz * k + g.vcst, pu1 = g.uz * k + g.ucst;
@@ -336,7 +356,7 @@ int orc_sb3d(double *out, const double *in, long nx, long ny, long nz,
                     for (m = 0; m < npv; ++m)
                         for (l = 0; l < npu; ++l)
                             acc += rho0 * wu[l] * wv[m] * in[base + m * nu + l];
-                    out[(k * ny + j) * nx + i] += acc;
+                    out[((k - k0) * ny + j) * nx + i] += acc;
                 }
             }
         }
@@ -771,4 +791,112 @@ double orc_sb3d_count_pairs(long nx, long ny, long nz, double s, int deg,
                 if (ul < ur && vl < vr) total += (ur - ul) * (vr - vl);
             }
     return total;
+}
+
+/* Back projection of `nview` views restricted to the slices k0 <= k < k1:
+ * vol[(k-k0)][ny][nx] += sum_view A_view' sino[view].  OpenMP over views,
+ * each thread summing into a private slab that is added at the end (the
+ * summation order of a view's contributions is that of orc_sb3d; the order
+ * over views depends on the thread schedule, within FP64 rounding). */
+int orc_sb3d_views_bwd(double *vol, const double *sino, long nx, long ny,
+                       long nz, double s, int deg, int kind, const double *cum,
+                       long m, double step, double coord0, long nu, long nv,
+                       double us, double vs, const double *xos,
+                       const double *xsd, long nview, int cone, long k0,
+                       long k1)
+{
+    orc_fp_t fp;
+    long nslab = (k1 - k0) * ny * nx;
+    int bad = 0;
+    if (k0 < 0 || k1 > nz || k0 >= k1) return ORC_FAIL;
+    fp.kind = kind; fp.cum = cum; fp.m = m; fp.step = step; fp.coord0 = coord0;
+#pragma omp parallel reduction(| : bad)
+    {
+        long v, q;
+        double *acc = (double *)calloc((size_t)nslab, sizeof(double));
+        if (acc == NULL) bad = 1;
+#pragma omp for schedule(dynamic, 1)
+        for (v = 0; v < nview; ++v)
+            if (acc != NULL)
+                bad |= sb3d_slab(acc, sino + v * nu * nv, nx, ny, nz, s, deg, &fp,
+                                 nu, nv, us, vs, xos + 12 * v, xsd + 12 * v,
+                                 cone, 1, k0, k1) != ORC_OK;
+        if (acc != NULL) {
+#pragma omp critical
+            for (q = 0; q < nslab; ++q) vol[q] += acc[q];
+            free(acc);
+        }
+    }
+    return bad ? ORC_FAIL : ORC_OK;
+}
+
+/* 2-D projector over `nview` views (OpenMP over views).  job 0:
+ * sino[view][nv] += A_view img; job != 0: img += sum_view A_view' sino[view]
+ * (private per-thread images added at the end). */
+int orc_sb2d_views(double *out, const double *in, long nx, long ny, double s,
+                   int deg, int kind, const double *cum, long m, double step,
+                   double coord0, long nv, double vs, const double *xos,
+                   const double *xsd, long nview, int fan, int job)
+{
+    orc_fp_t fp;
+    int bad = 0;
+    fp.kind = kind; fp.cum = cum; fp.m = m; fp.step = step; fp.coord0 = coord0;
+    if (job == 0) {
+        long v;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+        for (v = 0; v < nview; ++v)
+            bad |= orc_sb2d(out + v * nv, in, nx, ny, s, deg, &fp, nv, vs,
+                            xos + 12 * v, xsd + 12 * v, fan, 0) != ORC_OK;
+        return bad ? ORC_FAIL : ORC_OK;
+    }
+#pragma omp parallel reduction(| : bad)
+    {
+        long v, q;
+        double *acc = (double *)calloc((size_t)(nx * ny), sizeof(double));
+        if (acc == NULL) bad = 1;
+#pragma omp for schedule(dynamic, 4)
+        for (v = 0; v < nview; ++v)
+            if (acc != NULL)
+                bad |= orc_sb2d(acc, in + v * nv, nx, ny, s, deg, &fp, nv, vs,
+                                xos + 12 * v, xsd + 12 * v, fan, 1) != ORC_OK;
+        if (acc != NULL) {
+#pragma omp critical
+            for (q = 0; q < nx * ny; ++q) out[q] += acc[q];
+            free(acc);
+        }
+    }
+    return bad ? ORC_FAIL : ORC_OK;
+}
+
+/* Long & Fessler projector over `nview` views (beta[view], offset[view]).
+ * job 0: sino[view][nd] = A_view img (each view's row cleared first);
+ * job != 0: img += sum_view A_view' sino[view]. */
+int orc_lf2d_views(const orc_lf2d_t *p, const double *beta,
+                   const double *offset, const double *src, double *dst,
+                   long nview, int job)
+{
+    long nd = p->detectorDimension, npix = p->objectDimension1 * p->objectDimension2;
+    int bad = 0;
+    if (job == 0) {
+        long v;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (v = 0; v < nview; ++v)
+            orc_lf2d(p, beta[v], offset[v], src, dst + v * nd, 1, 0);
+        return ORC_OK;
+    }
+#pragma omp parallel reduction(| : bad)
+    {
+        long v, q;
+        double *acc = (double *)calloc((size_t)npix, sizeof(double));
+        if (acc == NULL) bad = 1;
+#pragma omp for schedule(dynamic, 4)
+        for (v = 0; v < nview; ++v)
+            if (acc != NULL) orc_lf2d(p, beta[v], offset[v], src + v * nd, acc, 0, 1);
+        if (acc != NULL) {
+#pragma omp critical
+            for (q = 0; q < npix; ++q) dst[q] += acc[q];
+            free(acc);
+        }
+    }
+    return bad ? ORC_FAIL : ORC_OK;
 }

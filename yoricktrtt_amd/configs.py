@@ -108,6 +108,7 @@ def _solve(obj, x0, iters, truth=None, sample=None):
     t0 = _sync()
     r = vmlmb(obj, x0, lower=0.0, maxiter=iters)
     t = _sync() - t0
+    obj.data.check()
     out = dict(iters=r.niter, evals=r.neval, status=r.status, solve_s=t,
                iter_ms=1e3 * t / max(r.niter, 1), f0=r.history[0], f=r.f)
     if truth is not None:

@@ -283,8 +283,21 @@ class Projector:
             raise RuntimeError(_lib.last_error())
         return vol
 
+    def _check_fx(self, fx):
+        torch = self.torch
+        if (not isinstance(fx, torch.Tensor) or not fx.is_cuda or fx.dtype != torch.float64
+                or fx.numel() != 1):
+            raise ValueError("fx must be a 1-element float64 device tensor")
+
     def weighted_residual(self, sino, y, w, fx, stream=None):
-        """sino <- w*(sino-y) and fx += 0.5 sum w (sino-y)^2 (fx: 1-element float64)."""
+        """sino <- w*(sino-y) and fx += 0.5 sum w (sino-y)^2 (fx: 1-element float64).
+        sino, y and w (optional) are contiguous device tensors of the plan's
+        type and sinogram shape."""
+        self._check(sino, self.sino_shape)
+        self._check(y, self.sino_shape)
+        if w is not None:
+            self._check(w, self.sino_shape)
+        self._check_fx(fx)
         wp = C.c_void_p(w.data_ptr()) if w is not None else None
         if self.lib.trtt_hip_weighted_residual(self.handle, C.c_void_p(sino.data_ptr()),
                                                C.c_void_p(y.data_ptr()), wp,
@@ -402,6 +415,9 @@ class SparseMatrix:
 
     def weighted_residual(self, sino, y, w, fx, stream=None):
         self.proj.weighted_residual(sino, y, w, fx, stream)
+
+    def error_flag(self) -> int:
+        return self.proj.error_flag()
 
     def coo(self):
         n = self.nnz
@@ -595,6 +611,9 @@ class DynamicProjector:
 
     def weighted_residual(self, sino, y, w, fx, stream=None):
         self.P.weighted_residual(sino, y, w, fx, stream)
+
+    def error_flag(self) -> int:
+        return self.P.error_flag()
 
     def close(self):
         self.P.close()

@@ -43,9 +43,14 @@ class DataTerm:
 
         self.torch = torch
         self.P = projector
-        self.y = y.contiguous()
-        self.w = None if w is None else w.contiguous()
-        self.sino = torch.zeros(projector.sino_shape, dtype=projector.dtype, device=y.device)
+        shape = tuple(projector.sino_shape)
+        # data and weights in the plan's type (the residual kernel reads them as such)
+        self.y = y.to(dtype=projector.dtype).contiguous()
+        self.w = None if w is None else w.to(dtype=projector.dtype).contiguous()
+        for name, t in (("y", self.y), ("w", self.w)):
+            if t is not None and tuple(t.shape) != shape:
+                raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shape}")
+        self.sino = torch.zeros(shape, dtype=projector.dtype, device=y.device)
         self.fx = torch.zeros(1, dtype=torch.float64, device=y.device)
         self.graph = None
         self.x_static = None
@@ -59,8 +64,18 @@ class DataTerm:
         g.zero_()
         self.P.backward(self.sino, g)
 
+    def check(self) -> None:
+        """Raise when a launch met a voxel in the source plane, the case where
+        the reference projectors return TRTT_FAILURE (trtt_3D_projectors.c:
+        1264-1284); the device flag is sticky, so one check covers every
+        evaluation since the plan was made.  Synchronous."""
+        flag = getattr(self.P, "error_flag", None)
+        if flag is not None and flag() != 0:
+            raise RuntimeError("projector failure: a voxel lies in the source plane")
+
     def eval(self, x, g) -> float:
-        """Return f(x) (FP64) and write g(x) into g."""
+        """Return f(x) (FP64) and write g(x) into g.  Eager evaluations check
+        the projector's error flag; graph replays leave it to check()."""
         if self.graph is not None:
             if x.data_ptr() != self.x_static.data_ptr():
                 self.x_static.copy_(x)
@@ -69,7 +84,9 @@ class DataTerm:
                 g.copy_(self.g_static)
             return float(self.fx.item())
         self._run(x, g)
-        return float(self.fx.item())
+        f = float(self.fx.item())
+        self.check()
+        return f
 
     def capture(self, x_static, g_static):
         """Capture the whole evaluation (all launches) in one HIP graph with
@@ -81,6 +98,8 @@ class DataTerm:
         with torch.cuda.stream(s):
             self._run(x_static, g_static)  # warm-up outside the graph
         torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.check()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             self._run(x_static, g_static)
