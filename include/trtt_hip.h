@@ -123,6 +123,26 @@ int trtt_hip_DD_PB3D_adjoint(double *out, double *in, long nx, long ny, long nz,
                              long nu, long nv, double u_scl, double v_scl, double *xos,
                              double *xsd);
 
+/* Allocation-free variant of the eight footprint drop-ins above (no
+ * counterpart in the reference).  It is for callers that cannot allocate
+ * device memory: code run while a HIP graph is being set up, serving loops,
+ * harnesses that forbid hipMalloc.  kind is TRTT_HIP_PB3D, _CB3D, _PB2D or
+ * _FB2D; cum = NULL selects the closed-form cubic footprint (the *_cubic
+ * symbols), else the tabulated one (cum_footprint, size_footprint,
+ * step_footprint and coord_footprint[0] = coord0); the other arguments and
+ * the semantics (out += A in for job 0, out += A' in otherwise, precision of
+ * trtt_hip_set_precision) are those of the drop-ins.  It always runs the
+ * general footprint kernels.  Every device buffer lives in one static 64 MiB
+ * device array per GPU (held until the process ends): calls whose tables,
+ * staging and output need more fail with TRTT_FAILURE and a message naming
+ * the limit.  Calls on one GPU share that array; they are serialised within
+ * the process, and concurrent calls from several processes on one GPU each
+ * use their own copy. */
+int trtt_hip_dropin_noalloc(int kind, double *out, const double *in, long nx, long ny, long nz,
+                            double s_scl, int s_deg, const double *cum, long size_footprint,
+                            double step_footprint, double coord0, long nu, long nv, double u_scl,
+                            double v_scl, const double *xos, const double *xsd, int job);
+
 /* Sparse matrix-vector product out[i[k]-1] += a[k]*in[j[k]-1] (1-based indices).
  * Replaces plugin/trtt_common.c:62-69. */
 int trtt_mvmult(double *out, double *in, long number, double *a, long *i, long *j);

@@ -109,6 +109,7 @@ _SIGS = {
     "trtt_mvmult": ([dp, dp, L, dp, lp, lp], I),
     "trtt_DD_PB3D": ([dp, dp, L, L, L, D, L, L, D, D, dp, dp, I], I),
     "trtt_hip_DD_PB3D_adjoint": ([dp, dp, L, L, L, D, L, L, D, D, dp, dp], I),
+    "trtt_hip_dropin_noalloc": ([I, dp, dp, L, L, L, D, I, dp, L, D, D, L, L, D, D, dp, dp, I], I),
 }
 
 _lib = None

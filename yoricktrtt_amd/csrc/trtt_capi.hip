@@ -264,12 +264,14 @@ constexpr long kTargetBlocks = 4096;
 
 // Tiles of the general kernels (2-D: 1024 pixels, 16 x kTile2DRows in the
 // backward, 32 x 32 in the forward).
-int tiles_of(const trtt_hip_plan *p) {
-  const bool is2d = p->kind == trtt::PB2D || p->kind == trtt::FB2D || p->kind == trtt::LF2D;
+int tiles_of_dims(int kind, long nx, long ny, long nz) {
+  const bool is2d = kind == trtt::PB2D || kind == trtt::FB2D || kind == trtt::LF2D;
   const long tx = is2d ? 16 : 8, tz = is2d ? 1 : 8;
   const long ty = is2d ? trtt::kTile2DRows : 8;
-  return int(((p->nx + tx - 1) / tx) * ((p->ny + ty - 1) / ty) * ((p->nz + tz - 1) / tz));
+  return int(((nx + tx - 1) / tx) * ((ny + ty - 1) / ty) * ((nz + tz - 1) / tz));
 }
+
+int tiles_of(const trtt_hip_plan *p) { return tiles_of_dims(p->kind, p->nx, p->ny, p->nz); }
 
 // Group views by (t0, t1) and cut the ordered list into `nrows` back-
 // projection rows of contiguous segments.
@@ -1489,6 +1491,197 @@ void lf2d_direct(const lf2d_parameters_t *p, double beta, double detectorOffset,
 void lf2d_adjoint(const lf2d_parameters_t *p, double beta, double detectorOffset,
                   const double pixel[], double voxel[], int clr) {
   lf_dropin(p, beta, detectorOffset, pixel, voxel, clr, 1);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free drop-in (no reference counterpart).  Callers that cannot
+// allocate device memory (code under HIP graph capture set-up, serving loops,
+// harnesses that forbid hipMalloc) get the per-view footprint projectors of
+// the drop-in symbols with every device buffer (view constants, table,
+// staging, output) carved out of one static device array per GPU.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = size_t(64) << 20;  // 64 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+// Device address of `scratch` on each GPU (hipGetSymbolAddress, once per
+// device); the array's name is only a host-side stand-in.
+unsigned char *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+unsigned char *noalloc_base(int dev) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    fail("device index out of range for the allocation-free entry point");
+    return nullptr;
+  }
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress"))
+      return nullptr;
+    g_noalloc_ptr[dev] = static_cast<unsigned char *>(ptr);
+  }
+  return g_noalloc_ptr[dev];
+}
+
+__global__ void zero_bytes(unsigned char *__restrict__ p, long n) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+template <typename T>
+__global__ void add_into_f64(const T *__restrict__ a, double *__restrict__ b, long n) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    b[i] += double(a[i]);
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+int trtt_hip_dropin_noalloc(int kind, double *out, const double *in, long nx, long ny, long nz,
+                            double s_scl, int s_deg, const double *cum, long size_footprint,
+                            double step_footprint, double coord0, long nu, long nv, double u_scl,
+                            double v_scl, const double *xos, const double *xsd, int job) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (kind < trtt::PB3D || kind > trtt::FB2D) return fail("unknown projector kind");
+  if (!out || !in || !xos || !xsd) return fail("null argument");
+  const bool is2d = kind == trtt::PB2D || kind == trtt::FB2D;
+  if (is2d) {
+    nz = 1;
+    nu = 1;
+  }
+  if (bad_sizes(nx, ny, nz, nu, nv) || (cum && size_footprint < 2))
+    return fail("invalid projector arguments");
+  const bool table = cum != nullptr, f64 = g_precision == TRTT_HIP_F64;
+  const size_t esz = f64 ? 8 : 4;
+  const long nvol = nx * ny * nz, nsino = nu * nv;
+  const long nin = job == 0 ? nvol : nsino, nout = job == 0 ? nsino : nvol;
+  const long m = table ? size_footprint : 0;
+  // layout: error flag | view constants | order, segment, row bounds |
+  //         table | FP64 staging | input | output
+  const size_t o_view = 256, o_idx = o_view + align256(sizeof(ViewConst));
+  const size_t o_table = o_idx + 256;
+  const size_t o_stage = o_table + align256(esz * size_t(m));
+  const size_t o_in = o_stage + align256(8 * size_t(std::max(std::max(nin, nout), m)));
+  const size_t o_out = o_in + align256(esz * size_t(nin));
+  const size_t total = o_out + align256(esz * size_t(nout));
+  if (total > kNoAllocBytes)
+    return fail("allocation-free projector: " + std::to_string(total) +
+                " bytes of device scratch needed, the static array holds " +
+                std::to_string(kNoAllocBytes));
+  if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return TRTT_FAILURE;
+  unsigned char *base = noalloc_base(g_device);
+  if (!base) return TRTT_FAILURE;
+  int *d_err = reinterpret_cast<int *>(base);
+  ViewConst *d_view = reinterpret_cast<ViewConst *>(base + o_view);
+  int *d_order = reinterpret_cast<int *>(base + o_idx);
+  trtt::Segment *d_seg = reinterpret_cast<trtt::Segment *>(base + o_idx + 64);
+  int *d_seg_begin = reinterpret_cast<int *>(base + o_idx + 128);
+  void *d_table = base + o_table;
+  double *d_stage = reinterpret_cast<double *>(base + o_stage);
+  void *d_in = base + o_in, *d_out = base + o_out;
+
+  // per-view constants and the one-view schedule (as make_plan builds them)
+  ViewConst g;
+  const double step = table ? step_footprint : 1.0, c0 = table ? coord0 : 0.0;
+  if (is2d)
+    view_consts_2d(g, xos, xsd, kind == trtt::FB2D, table, step, s_scl, s_deg, v_scl);
+  else
+    view_consts_3d(g, xos, xsd, kind == trtt::CB3D, s_scl, s_deg, u_scl, v_scl);
+  g.t0 = g.t1 = 0;
+  g.w0 = 1.0;
+  g.w1 = 0.0;
+  const int order = 0;
+  const trtt::Segment seg{0, 1, 0, 0};
+  const int seg_begin[2] = {0, 1};
+  hipStream_t st = nullptr;
+  hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(64), 0, st, base, long(sizeof(int)));
+  if (!hip_ok(hipMemcpy(d_view, &g, sizeof(g), hipMemcpyHostToDevice), "hipMemcpy") ||
+      !hip_ok(hipMemcpy(d_order, &order, sizeof(order), hipMemcpyHostToDevice), "hipMemcpy") ||
+      !hip_ok(hipMemcpy(d_seg, &seg, sizeof(seg), hipMemcpyHostToDevice), "hipMemcpy") ||
+      !hip_ok(hipMemcpy(d_seg_begin, seg_begin, sizeof(seg_begin), hipMemcpyHostToDevice),
+              "hipMemcpy"))
+    return TRTT_FAILURE;
+  if (table) {
+    if (f64) {
+      if (!hip_ok(hipMemcpy(d_table, cum, sizeof(double) * m, hipMemcpyHostToDevice), "hipMemcpy"))
+        return TRTT_FAILURE;
+    } else {
+      if (!hip_ok(hipMemcpy(d_stage, cum, sizeof(double) * m, hipMemcpyHostToDevice), "hipMemcpy"))
+        return TRTT_FAILURE;
+      hipLaunchKernelGGL(cvt_d2f, dim3(grid_for(m)), dim3(256), 0, st, d_stage,
+                         static_cast<float *>(d_table), m);
+    }
+  }
+  if (f64) {
+    if (!hip_ok(hipMemcpy(d_in, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
+      return TRTT_FAILURE;
+  } else {
+    if (!hip_ok(hipMemcpy(d_stage, in, sizeof(double) * nin, hipMemcpyHostToDevice), "hipMemcpy"))
+      return TRTT_FAILURE;
+    hipLaunchKernelGGL(cvt_d2f, dim3(grid_for(nin)), dim3(256), 0, st, d_stage,
+                       static_cast<float *>(d_in), nin);
+  }
+  hipLaunchKernelGGL(zero_bytes, dim3(grid_for(long(esz) * nout)), dim3(256), 0, st,
+                     static_cast<unsigned char *>(d_out), long(esz) * nout);
+
+  trtt::DevPlan P{};
+  P.views = d_view;
+  P.order = d_order;
+  P.segs = d_seg;
+  P.table = table ? d_table : nullptr;
+  P.err = d_err;
+  P.m = m;
+  P.step = step;
+  P.coord0 = c0;
+  P.nx = int(nx);
+  P.ny = int(ny);
+  P.nz = int(nz);
+  P.nu = int(nu);
+  P.nv = int(nv);
+  P.nviews = 1;
+  P.nt = 1;
+  P.vox_per_phase = nvol;
+  P.u_scl = u_scl;
+  P.v_scl = v_scl;
+  {
+#pragma clang fp contract(off)
+    P.rho_init = is2d ? s_scl * s_scl : s_scl * s_scl * s_scl;
+    P.hu_init = s_scl / u_scl;
+    P.hv_init = s_scl / v_scl;
+    P.hs0 = 0.5 * (s_deg + 1);
+  }
+  const bool quirk = is2d && !table;
+  const long ntiles = tiles_of_dims(kind, nx, ny, nz);
+  const int vpb = int(std::max(1L, std::min(ntiles / kTargetBlocks, 8L)));
+  if (!hip_ok(trtt::launch_projector(kind, table, quirk, f64, P, job == 0 ? 0 : 1, d_in, d_out,
+                                     d_seg_begin, 1, job == 0 ? vpb : 1, 0, st),
+              "kernel launch"))
+    return TRTT_FAILURE;
+  // out_host + result: the host output is staged in FP64 and the result
+  // added to it on the device
+  if (!hip_ok(hipMemcpy(d_stage, out, sizeof(double) * nout, hipMemcpyHostToDevice), "hipMemcpy"))
+    return TRTT_FAILURE;
+  if (f64)
+    hipLaunchKernelGGL(add_into_f64<double>, dim3(grid_for(nout)), dim3(256), 0, st,
+                       static_cast<const double *>(d_out), d_stage, nout);
+  else
+    hipLaunchKernelGGL(add_into_f64<float>, dim3(grid_for(nout)), dim3(256), 0, st,
+                       static_cast<const float *>(d_out), d_stage, nout);
+  if (!hip_ok(hipGetLastError(), "kernel launch")) return TRTT_FAILURE;
+  int err = 0;
+  if (!hip_ok(hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy"))
+    return TRTT_FAILURE;
+  if (err) return fail("IT WAS SUPPOSED TO BE HIGHLY IMPOSSIBLE: a voxel lies in the source plane");
+  if (!hip_ok(hipMemcpy(out, d_stage, sizeof(double) * nout, hipMemcpyDeviceToHost), "hipMemcpy"))
+    return TRTT_FAILURE;
+  return TRTT_SUCCESS;
 }
 
 }  // extern "C"
