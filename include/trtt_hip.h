@@ -54,7 +54,7 @@ int trtt_FB2D_cubic(double *out, double *in, long nx, long ny, double s_scl, int
                     long nv, double v_scl, double *xos, double *xsd, int job);
 
 /* 3-D parallel / cone beam, tabulated footprint CDF.
- * Replaces plugin/trtt_3D_projectors.h:396, :520 (trtt_3D_projectors.c:959, :1179). */
+ * Replaces plugin/trtt_3D_projectors.h:299, :423 (trtt_3D_projectors.c:959, :1179). */
 int trtt_PB3D(double *out, double *in, long nx, long ny, long nz, double s_scl, int s_deg,
               double *footprint, double *cum_footprint, long size_footprint,
               double step_footprint, double *coord_footprint, long nu, long nv,
@@ -87,6 +87,44 @@ void lf2d_direct(const lf2d_parameters_t *p, double beta, double detectorOffset,
                  const double voxel[], double pixel[], int clr);
 void lf2d_adjoint(const lf2d_parameters_t *p, double beta, double detectorOffset,
                   const double pixel[], double voxel[], int clr);
+
+/* Line-integral (point-sampled trapezoid) Long & Fessler model: the
+ * footprint of a pixel is sampled at each bin centre instead of integrated
+ * over the bin, with amplitude sqrt(1+q^2) voxelSize.  GPU, like
+ * lf2d_direct / lf2d_adjoint.
+ * Replaces tomography-LongFessler-model/LongFessler2D.h:40-49, 61-71
+ * (LongFessler2D.c:143 lf2d_line_integ_direct, :357 lf2d_line_integ_adjoint). */
+void lf2d_line_integ_direct(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                            const double voxel[], double pixel[], int clr);
+void lf2d_line_integ_adjoint(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                             const double pixel[], double voxel[], int clr);
+
+/* Coefficient column of one object pixel (i1, i2) on one view:
+ * pixel[k] += coefficient of bin k for a unit pixel value (clr: pixel zeroed
+ * first).  Host code with the kernels' own footprint functions (one pixel is
+ * a few dozen operations); trtt_hip_sparse_create builds the whole matrix on
+ * the device.  Glue: tomography-LongFessler-model/ywrap.c:55-77.
+ * Replaces LongFessler2D.c:472 (lf2d_sparser) and :573
+ * (lf2d_line_integ_sparser). */
+void lf2d_sparser(const lf2d_parameters_t *p, double beta, double detectorOffset, const long i1,
+                  const long i2, double pixel[], int clr);
+void lf2d_line_integ_sparser(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                             const long i1, const long i2, double pixel[], int clr);
+
+/* Coefficient column of image pixel (i, j) on one view with the tabulated
+ * footprint: out[bin] += coefficient for a unit pixel value (nx, ny and
+ * footprint are not read, as in the reference).  Host code, as for
+ * lf2d_sparser.  Glue: plugin/ywrap.c:295-321.
+ * Replaces plugin/trtt_2D_projectors.h:80-82 (trtt_2D_projectors.c:393
+ * trtt_PB2D_sparser, :454 trtt_FB2D_sparser). */
+int trtt_PB2D_sparser(double *out, long i, long j, long nx, long ny, double s_scl, int s_deg,
+                      double *footprint, double *cum_footprint, long size_footprint,
+                      double step_footprint, double *coord_footprint, long nv, double v_scl,
+                      double *xos, double *xsd);
+int trtt_FB2D_sparser(double *out, long i, long j, long nx, long ny, double s_scl, int s_deg,
+                      double *footprint, double *cum_footprint, long size_footprint,
+                      double step_footprint, double *coord_footprint, long nv, double v_scl,
+                      double *xos, double *xsd);
 
 /* Host-side affine transforms used by the Yorick XForm3D object
  * (plugin/trtt_xform3d.h; plugin/Ytrtt.c:9-116).  Row-major 3x4 storage. */
@@ -196,6 +234,11 @@ trtt_hip_plan_t *trtt_hip_plan_create(int kind, int precision, long nx, long ny,
 trtt_hip_plan_t *trtt_hip_plan_create_lf2d(int precision, const lf2d_parameters_t *p,
                                            const double *beta, const double *offset,
                                            long nviews);
+
+/* Long & Fessler plans: line = 1 selects the line-integral model of
+ * lf2d_line_integ_direct / _adjoint (and its sparse matrix), 0 the
+ * bin-integrated one (default). */
+int trtt_hip_plan_set_lf_line(trtt_hip_plan_t *plan, int line);
 
 /* sino[view] += A_view vol for every view (device pointers, plan precision).
  * `stream` is a hipStream_t (NULL: the default stream).  Asynchronous. */

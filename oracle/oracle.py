@@ -71,7 +71,9 @@ class Oracle:
         lib.orc_sb3d_views_bwd.restype = I
         lib.orc_sb2d_views.argtypes = [dp, dp, L, L, D, I, I, dp, L, D, D, L, D, dp, dp, L, I, I]
         lib.orc_sb2d_views.restype = I
-        lib.orc_lf2d_views.argtypes = [C.POINTER(LF2DParams), dp, dp, dp, dp, L, I]
+        lib.orc_lf2d_views.argtypes = [C.POINTER(LF2DParams), dp, dp, dp, dp, L, I, I]
+        lib.orc_lf2d_ex.argtypes = [C.POINTER(LF2DParams), D, D, dp, dp, I, I, I]
+        lib.orc_lf2d_ex.restype = None
         lib.orc_lf2d_views.restype = I
         lib.orc_sb3d_count_pairs.argtypes = [L, L, L, D, I, L, L, D, D, dp, dp, I]
         lib.orc_sb3d_count_pairs.restype = D
@@ -161,7 +163,7 @@ class Oracle:
             raise RuntimeError("oracle projector failure")
         return out
 
-    def lf2d_views(self, params: LF2DParams, beta, offset, src, job):
+    def lf2d_views(self, params: LF2DParams, beta, offset, src, job, line=False):
         """Long & Fessler projector over every view: job 0 returns the
         sinogram [view][nd] of image ``src``; job 1 the back projection."""
         beta, offset = _f64(beta), _f64(offset)
@@ -169,7 +171,7 @@ class Oracle:
         out = (np.zeros((nview, nd)) if job == 0
                else np.zeros((params.objectDimension2, params.objectDimension1)))
         r = self.lib.orc_lf2d_views(C.byref(params), _p(beta), _p(offset), _p(_f64(src)), _p(out),
-                                    nview, int(job))
+                                    nview, int(job), int(line))
         if r != 0:
             raise RuntimeError("oracle projector failure")
         return out
@@ -187,11 +189,13 @@ class Oracle:
                              int(job), int(adjoint))
         return out
 
-    def lf2d(self, params: LF2DParams, beta, offset, src, dst, clr, job):
+    def lf2d(self, params: LF2DParams, beta, offset, src, dst, clr, job, line=False):
+        """lf2d_direct / lf2d_adjoint (job 0 / 1); ``line`` selects the
+        line-integral model (lf2d_line_integ_direct / _adjoint)."""
         src = _f64(src)
         assert dst.dtype == np.float64 and dst.flags.c_contiguous
-        self.lib.orc_lf2d(C.byref(params), float(beta), float(offset), _p(src), _p(dst), int(clr),
-                          int(job))
+        self.lib.orc_lf2d_ex(C.byref(params), float(beta), float(offset), _p(src), _p(dst),
+                             int(clr), int(job), int(line))
         return dst
 
     @staticmethod

@@ -573,8 +573,41 @@ static void sort4(double t[4])
 /* lf2d_direct (job 0, LongFessler2D.c:41-141) and lf2d_adjoint (job 1,
  * :254-355).  Image [objectDimension2][objectDimension1], detector
  * [detectorDimension]; `clr` zeroes the destination first. */
+/* Point-sampled trapezoid (the line-integral model of
+ * lf2d_line_integ_direct / _adjoint / _sparser, LongFessler2D.c:143-252,
+ * 357-470, 573-681): the height at bin centre k of the trapezoid rising over
+ * [t0,t1], flat to t2, falling to t3, times a. */
+static double lf_point(const double t[4], long k, double a)
+{
+    double x = (double)k;
+    if (x >= t[0] && x < t[1]) return a * (x - t[0]) / (t[1] - t[0]);
+    if (x >= t[1] && x < t[2]) return a;
+    if (x >= t[2] && x < t[3]) return a * (x - t[3]) / (t[2] - t[3]);
+    return 0.0;
+}
+
+static void lf2d_model(const orc_lf2d_t *p, double beta, double det_offset,
+                       const double *src, double *dst, int clr, int job,
+                       int line);
+
 void orc_lf2d(const orc_lf2d_t *p, double beta, double det_offset,
               const double *src, double *dst, int clr, int job)
+{
+    lf2d_model(p, beta, det_offset, src, dst, clr, job, 0);
+}
+
+/* line = 0: lf2d_direct / lf2d_adjoint (bin-integrated trapezoid);
+ * line = 1: lf2d_line_integ_direct / _adjoint (trapezoid sampled at the bin
+ * centres, amplitude sqrt(1+q^2) voxelSize). */
+void orc_lf2d_ex(const orc_lf2d_t *p, double beta, double det_offset,
+                 const double *src, double *dst, int clr, int job, int line)
+{
+    lf2d_model(p, beta, det_offset, src, dst, clr, job, line);
+}
+
+static void lf2d_model(const orc_lf2d_t *p, double beta, double det_offset,
+                       const double *src, double *dst, int clr, int job,
+                       int line)
 {
     long n1 = p->objectDimension1, n2 = p->objectDimension2;
     long nd = p->detectorDimension, i1, i2, k;
@@ -604,13 +637,17 @@ void orc_lf2d(const orc_lf2d_t *p, double beta, double det_offset,
             qx = x + R * sb;
             qy = y - R * cb;
             q = fabs(qy) > fabs(qx) ? qx / qy : qy / qx;
-            amp = sqrt(1.0 + q * q) * (vs * ps);
+            amp = line ? sqrt(1.0 + q * q) * vs : sqrt(1.0 + q * q) * (vs * ps);
             if (job == 0) amp *= src[i1 + i2 * n1];
             kf = (long)floor(t[0] - 0.5 * size);
             if (kf < 0) kf = 0;
             kl = (long)ceil(t[3] + 0.5 * size);
             if (kl >= nd) kl = nd - 1;
-            for (k = kf; k <= kl; ++k) {
+            for (k = kf; k <= kl && line; ++k) {
+                if (job == 0) dst[k] += lf_point(t, k, amp);
+                else acc += lf_point(t, k, amp) * src[k];
+            }
+            for (k = kf; k <= kl && !line; ++k) {
                 double s1 = k - 0.5 * size, s2 = s1 + size;
                 double lo1 = s1 > t[0] ? s1 : t[0], hi1 = s2 < t[1] ? s2 : t[1];
                 double lo2 = s1 > t[1] ? s1 : t[1], hi2 = s2 < t[2] ? s2 : t[2];
@@ -873,7 +910,7 @@ int orc_sb2d_views(double *out, const double *in, long nx, long ny, double s,
  * job != 0: img += sum_view A_view' sino[view]. */
 int orc_lf2d_views(const orc_lf2d_t *p, const double *beta,
                    const double *offset, const double *src, double *dst,
-                   long nview, int job)
+                   long nview, int job, int line)
 {
     long nd = p->detectorDimension, npix = p->objectDimension1 * p->objectDimension2;
     int bad = 0;
@@ -881,7 +918,7 @@ int orc_lf2d_views(const orc_lf2d_t *p, const double *beta,
         long v;
 #pragma omp parallel for schedule(dynamic, 4)
         for (v = 0; v < nview; ++v)
-            orc_lf2d(p, beta[v], offset[v], src, dst + v * nd, 1, 0);
+            lf2d_model(p, beta[v], offset[v], src, dst + v * nd, 1, 0, line);
         return ORC_OK;
     }
 #pragma omp parallel reduction(| : bad)
@@ -891,7 +928,8 @@ int orc_lf2d_views(const orc_lf2d_t *p, const double *beta,
         if (acc == NULL) bad = 1;
 #pragma omp for schedule(dynamic, 4)
         for (v = 0; v < nview; ++v)
-            if (acc != NULL) orc_lf2d(p, beta[v], offset[v], src + v * nd, acc, 0, 1);
+            if (acc != NULL)
+                lf2d_model(p, beta[v], offset[v], src + v * nd, acc, 0, 1, line);
         if (acc != NULL) {
 #pragma omp critical
             for (q = 0; q < npix; ++q) dst[q] += acc[q];

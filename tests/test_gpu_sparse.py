@@ -107,17 +107,21 @@ def test_sparse_entries_cubic_vs_oracle(lib, oracle, kind):
     P.close()
 
 
-def test_sparse_entries_lf2d_vs_oracle(lib, oracle):
+@pytest.mark.parametrize("line", [False, True])
+def test_sparse_entries_lf2d_vs_oracle(lib, oracle, line):
+    """Long & Fessler matrix (lf2d_sparser, or lf2d_line_integ_sparser for
+    the line-integral model) against the oracle's one-hot columns."""
     from oracle.oracle import LF2DParams
     from yoricktrtt_amd._lib import LF2DParameters
     from yoricktrtt_amd.projectors import LFProjector
 
-    _say("sparse entries lf2d")
+    _say("sparse entries lf2d", "line" if line else "")
     n1, n2, nd = 15, 13, 41
     args = (0.9, 1.0, 1.3, 60.0, 40.0, 0.7, -1.1)
     beta = np.array([0.0, 0.9, 2.5, 5.1])
     off = np.array([1.5, 0.0, -0.5, 0.2])
-    P = LFProjector(LF2DParameters(*args, n1, n2, nd), beta, off, precision="f64")
+    P = LFProjector(LF2DParameters(*args, n1, n2, nd), beta, off, precision="f64",
+                    line_integ=line)
     M = P.sparse_matrix(6)
     coefs, rows, cols = M.coo()
     po = LF2DParams(*args, n1, n2, nd)
@@ -125,7 +129,7 @@ def test_sparse_entries_lf2d_vs_oracle(lib, oracle):
     def column(v, i, j):
         img = np.zeros((n2, n1))
         img[j, i] = 1.0
-        return oracle.lf2d(po, beta[v], off[v], img, np.zeros(nd), 1, 0)
+        return oracle.lf2d(po, beta[v], off[v], img, np.zeros(nd), 1, 0, line=line)
     rc, rr, rcol = oracle.sparse2d_coo(column, beta.size, n1, n2, 6)
     assert np.array_equal(rows, rr) and np.array_equal(cols, rcol)
     diff = np.abs(coefs - rc)

@@ -50,8 +50,15 @@ _SIGS = {
     "trtt_PB3D_cubic": (_CUB3, I), "trtt_CB3D_cubic": (_CUB3, I),
     "trtt_PB2D": (_TAB2, I), "trtt_FB2D": (_TAB2, I),
     "trtt_PB2D_cubic": (_CUB2, I), "trtt_FB2D_cubic": (_CUB2, I),
+    "trtt_PB2D_sparser": ([dp, L, L, L, L, D, I, dp, dp, L, D, dp, L, D, dp, dp], I),
+    "trtt_FB2D_sparser": ([dp, L, L, L, L, D, I, dp, dp, L, D, dp, L, D, dp, dp], I),
     "lf2d_direct": ([C.POINTER(LF2DParameters), D, D, dp, dp, I], None),
     "lf2d_adjoint": ([C.POINTER(LF2DParameters), D, D, dp, dp, I], None),
+    "lf2d_line_integ_direct": ([C.POINTER(LF2DParameters), D, D, dp, dp, I], None),
+    "lf2d_line_integ_adjoint": ([C.POINTER(LF2DParameters), D, D, dp, dp, I], None),
+    "lf2d_sparser": ([C.POINTER(LF2DParameters), D, D, L, L, dp, I], None),
+    "lf2d_line_integ_sparser": ([C.POINTER(LF2DParameters), D, D, L, L, dp, I], None),
+    "trtt_hip_plan_set_lf_line": ([vp, I], I),
     "trtt_hip_set_device": ([I], I),
     "trtt_hip_set_precision": ([I], I),
     "trtt_hip_last_error": ([], C.c_char_p),

@@ -41,6 +41,8 @@ hipError_t sparse_spmv(bool f64, const int *ptr, const int *idx, const void *val
                        void *y, int n_major, hipStream_t st);
 hipError_t launch_dd(bool f64, const DDConst *cs, const DDConst &c0, int nviews, int job,
                      int stuck, const void *in, void *out, hipStream_t st);
+bool host_column_2d(bool fan, const DevPlan &P, const ViewConst &g, int i, int j, double *out);
+void host_lf_column(const DevPlan &P, const ViewConst &g, int i1, int i2, double *out);
 hipError_t launch_tinterp(bool f64, int job, const void *in, void *out, long npix, int nframes,
                           int taps, const int *idx, const void *w, const int *ptr,
                           const int *src, const void *wb, hipStream_t st);
@@ -1466,7 +1468,7 @@ int trtt_CB3D_cubic(double *out, double *in, long nx, long ny, long nz, double s
 }
 
 static void lf_dropin(const lf2d_parameters_t *lp, double beta, double offset, const double *src,
-                      double *dst, int clr, int job) {
+                      double *dst, int clr, int job, int line = 0) {
   std::lock_guard<std::mutex> lock(g_mutex);
   g_last_error.clear();
   if (!lp || !src || !dst) {
@@ -1479,6 +1481,7 @@ static void lf_dropin(const lf2d_parameters_t *lp, double beta, double offset, c
   if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return;
   trtt_hip_plan *p = trtt_hip_plan_create_lf2d(g_precision, lp, &beta, &offset, 1);
   if (!p) return;
+  p->dev.lf_line = line;
   run_dropin(p, dst, src, job);
   release(p);
 }
@@ -1491,6 +1494,117 @@ void lf2d_direct(const lf2d_parameters_t *p, double beta, double detectorOffset,
 void lf2d_adjoint(const lf2d_parameters_t *p, double beta, double detectorOffset,
                   const double pixel[], double voxel[], int clr) {
   lf_dropin(p, beta, detectorOffset, pixel, voxel, clr, 1);
+}
+
+void lf2d_line_integ_direct(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                            const double voxel[], double pixel[], int clr) {
+  lf_dropin(p, beta, detectorOffset, voxel, pixel, clr, 0, 1);
+}
+
+void lf2d_line_integ_adjoint(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                             const double pixel[], double voxel[], int clr) {
+  lf_dropin(p, beta, detectorOffset, pixel, voxel, clr, 1, 1);
+}
+
+// Per-pixel columns (host code: a call evaluates one pixel's footprint on
+// one view, a few dozen FP64 operations, with the kernels' own footprint
+// functions; the whole matrix is built on the device by
+// trtt_hip_sparse_create).
+static int sparser_2d(bool fan, double *out, long i, long j, double s_scl, int s_deg,
+                      const double *cum, long m, double step, const double *coord, long nv,
+                      double v_scl, const double *xos, const double *xsd) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!out || !cum || !coord || !xos || !xsd) return fail("null argument");
+  if (nv <= 0 || nv > (1L << 30) || m < 2 || i < INT_MIN || i > INT_MAX || j < INT_MIN ||
+      j > INT_MAX)
+    return fail("invalid sparser arguments");
+  ViewConst g;
+  view_consts_2d(g, xos, xsd, fan, true, step, s_scl, s_deg, v_scl);
+  trtt::DevPlan P{};
+  P.table = cum;
+  P.m = m;
+  P.step = step;
+  P.coord0 = coord[0];
+  P.nz = P.nu = P.nviews = P.nt = 1;
+  P.nv = int(nv);
+  P.u_scl = 1.0;
+  P.v_scl = v_scl;
+  {
+#pragma clang fp contract(off)
+    P.rho_init = s_scl * s_scl;
+    P.hu_init = s_scl;
+    P.hv_init = s_scl / v_scl;
+    P.hs0 = 0.5 * (s_deg + 1);
+  }
+  if (!trtt::host_column_2d(fan, P, g, int(i), int(j), out))
+    return fail("IT WAS SUPPOSED TO BE HIGHLY IMPOSSIBLE: a pixel lies in the source plane");
+  return TRTT_SUCCESS;
+}
+
+int trtt_PB2D_sparser(double *out, long i, long j, long nx, long ny, double s_scl, int s_deg,
+                      double *footprint, double *cum_footprint, long size_footprint,
+                      double step_footprint, double *coord_footprint, long nv, double v_scl,
+                      double *xos, double *xsd) {
+  (void)nx;
+  (void)ny;
+  (void)footprint;
+  return sparser_2d(false, out, i, j, s_scl, s_deg, cum_footprint, size_footprint,
+                    step_footprint, coord_footprint, nv, v_scl, xos, xsd);
+}
+
+int trtt_FB2D_sparser(double *out, long i, long j, long nx, long ny, double s_scl, int s_deg,
+                      double *footprint, double *cum_footprint, long size_footprint,
+                      double step_footprint, double *coord_footprint, long nv, double v_scl,
+                      double *xos, double *xsd) {
+  (void)nx;
+  (void)ny;
+  (void)footprint;
+  return sparser_2d(true, out, i, j, s_scl, s_deg, cum_footprint, size_footprint,
+                    step_footprint, coord_footprint, nv, v_scl, xos, xsd);
+}
+
+static void lf_sparser(const lf2d_parameters_t *lp, double beta, double offset, long i1, long i2,
+                       double *pixel, int clr, int line) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!lp || !pixel || lp->detectorDimension <= 0 || lp->detectorDimension > (1L << 30) ||
+      i1 < INT_MIN || i1 > INT_MAX || i2 < INT_MIN || i2 > INT_MAX) {
+    fail("invalid Long & Fessler sparser arguments");
+    return;
+  }
+  if (clr) std::memset(pixel, 0, sizeof(double) * lp->detectorDimension);
+  ViewConst g;
+  view_consts_lf(g, lp, beta, offset);
+  trtt::DevPlan P{};
+  P.nv = int(lp->detectorDimension);
+  P.lf_line = line;
+  {
+#pragma clang fp contract(off)
+    P.lf_vs = lp->voxelSize;
+    P.lf_ps = lp->pixelSize;
+    P.lf_R = lp->sourceDistance;
+    P.lf_scale = (lp->sourceDistance + lp->detectorDistance) / lp->pixelStep;
+    P.lf_size = lp->pixelSize / lp->pixelStep;
+  }
+  trtt::host_lf_column(P, g, int(i1), int(i2), pixel);
+}
+
+void lf2d_sparser(const lf2d_parameters_t *p, double beta, double detectorOffset, const long i1,
+                  const long i2, double pixel[], int clr) {
+  lf_sparser(p, beta, detectorOffset, i1, i2, pixel, clr, 0);
+}
+
+void lf2d_line_integ_sparser(const lf2d_parameters_t *p, double beta, double detectorOffset,
+                             const long i1, const long i2, double pixel[], int clr) {
+  lf_sparser(p, beta, detectorOffset, i1, i2, pixel, clr, 1);
+}
+
+int trtt_hip_plan_set_lf_line(trtt_hip_plan_t *p, int line) {
+  if (!p) return fail("null argument");
+  if (p->kind != trtt::LF2D) return fail("not a Long & Fessler plan");
+  p->dev.lf_line = line != 0;
+  return TRTT_SUCCESS;
 }
 
 }  // extern "C"

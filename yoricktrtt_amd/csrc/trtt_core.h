@@ -59,6 +59,7 @@ struct DevPlan {
   double hu_init, hv_init, rho_init, hs0;
   // Long & Fessler scalars
   double lf_vs, lf_ps, lf_R, lf_scale, lf_size;
+  int lf_line;             // 1: point-sampled (line-integral) trapezoid model
 };
 
 // Constants of one view of the distance-driven parallel-beam projector

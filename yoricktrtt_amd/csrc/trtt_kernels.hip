@@ -32,7 +32,7 @@ constexpr int kMaxW = 12;          // weights held in registers per chunk
 // G(t) = t(2/3 + t^2(t/8 - 1/3)) on [0,1], 1/2 - (2-t)^4/24 on [1,2].
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T cdf3(T x) {
+__host__ __device__ __forceinline__ T cdf3(T x) {
   T t = fabs(x);
   t = t < T(2) ? t : T(2);
   const T t2 = t * t;
@@ -56,7 +56,7 @@ struct Foot {
 // Window clamp of trtt_3D_projectors.c:1293-1307; the 2-D closed-form
 // projectors clamp the right bound to n-1 (trtt_2D_projectors_cubic.c:256-258).
 template <bool QUIRK>
-__device__ __forceinline__ void clamp_window(double c, double hs, int n, int &lo, int &hi) {
+__host__ __device__ __forceinline__ void clamp_window(double c, double hs, int n, int &lo, int &hi) {
 #pragma clang fp contract(off)
   const double l = c - hs, r = c + hs;
   const double nm1 = double(n - 1);
@@ -71,7 +71,7 @@ __device__ __forceinline__ void clamp_window(double c, double hs, int n, int &lo
 // Per-voxel footprint (FP64).  Returns false when the voxel lies in the
 // source plane (the reference's TRTT_FAILURE case).
 template <int KIND, bool TABLE, bool QUIRK>
-__device__ __forceinline__ bool voxel_footprint(const DevPlan &P, const ViewConst &g, int i, int j,
+__host__ __device__ __forceinline__ bool voxel_footprint(const DevPlan &P, const ViewConst &g, int i, int j,
                                                 int k, Foot &f) {
 #pragma clang fp contract(off)
   const double di = double(i), dj = double(j), dk = double(k);
@@ -134,7 +134,7 @@ __device__ __forceinline__ bool voxel_footprint(const DevPlan &P, const ViewCons
   return true;
 }
 
-__device__ __forceinline__ long table_index(double x, long m) {
+__host__ __device__ __forceinline__ long table_index(double x, long m) {
   double c = x >= 0.0 ? x : 0.0;
   c = c <= double(m - 1) ? c : double(m - 1);
   return long(floor(c));
@@ -152,7 +152,8 @@ struct AxisFoot {
   const T *cum;
   long m;
 
-  __device__ __forceinline__ void init(const DevPlan &P, int lo, double c, double h, double scl) {
+  __host__ __device__ __forceinline__ void init(const DevPlan &P, int lo, double c, double h,
+                                                double scl) {
 #pragma clang fp contract(off)
     if (TABLE) {
       cum = static_cast<const T *>(P.table);
@@ -172,7 +173,7 @@ struct AxisFoot {
     }
   }
   // weight of pixel lo+l (must be called with l = 0, 1, 2, ... in order)
-  __device__ __forceinline__ T next(int l) {
+  __host__ __device__ __forceinline__ T next(int l) {
 #pragma clang fp contract(off)
     T nxt;
     if (TABLE) {
@@ -567,17 +568,18 @@ __global__ __launch_bounds__(kThreads) void fp_backward(DevPlan P, const T *__re
 // ---------------------------------------------------------------------------
 struct LFFoot {
   double t[4];
-  double amp;
-  int kf, kl;  // inclusive bin range
+  double amp;       // sqrt(1 + q^2) voxelSize pixelSize: the trapezoid's integral model
+  double amp_line;  // sqrt(1 + q^2) voxelSize: the point-sampled (line-integral) model
+  int kf, kl;       // inclusive bin range
 };
 
-__device__ __forceinline__ double lf_ramp(double a, double b, double c) {
+__host__ __device__ __forceinline__ double lf_ramp(double a, double b, double c) {
   return a < b ? (b + a) * (b - a) / (2.0 * c) : 0.0;
 }
-__device__ __forceinline__ double lf_box(double a, double b) { return a < b ? b - a : 0.0; }
+__host__ __device__ __forceinline__ double lf_box(double a, double b) { return a < b ? b - a : 0.0; }
 
-__device__ __forceinline__ void lf_footprint(const DevPlan &P, const ViewConst &g, int i1, int i2,
-                                             LFFoot &f) {
+__host__ __device__ __forceinline__ void lf_footprint(const DevPlan &P, const ViewConst &g, int i1,
+                                                      int i2, LFFoot &f) {
 #pragma clang fp contract(off)
   const double vs = P.lf_vs;
   const double y = g.lf_y0 + double(i2) * vs, ya = y - 0.5 * vs, yb = y + 0.5 * vs;
@@ -599,19 +601,34 @@ __device__ __forceinline__ void lf_footprint(const DevPlan &P, const ViewConst &
   const double qx = x + R * sb, qy = y - R * cb;
   const double q = fabs(qy) > fabs(qx) ? qx / qy : qy / qx;
   f.amp = sqrt(1.0 + q * q) * (vs * P.lf_ps);
+  f.amp_line = sqrt(1.0 + q * q) * vs;
   const double lo = floor(f.t[0] - 0.5 * P.lf_size);
   const double hi = ceil(f.t[3] + 0.5 * P.lf_size);
   f.kf = lo < 0.0 ? 0 : (lo > double(P.nv) ? P.nv : int(lo));
   f.kl = hi >= double(P.nv) ? P.nv - 1 : (hi < -1.0 ? -1 : int(hi));
 }
 
-__device__ __forceinline__ double lf_gamma(const LFFoot &f, int k, double size) {
+__host__ __device__ __forceinline__ double lf_gamma(const LFFoot &f, int k, double size) {
 #pragma clang fp contract(off)
   const double s1 = double(k) - 0.5 * size, s2 = s1 + size;
   const double *t = f.t;
   return lf_ramp((s1 >= t[0] ? s1 : t[0]) - t[0], (s2 <= t[1] ? s2 : t[1]) - t[0], t[1] - t[0]) +
          lf_box(s1 >= t[1] ? s1 : t[1], s2 <= t[2] ? s2 : t[2]) +
          lf_ramp(t[3] - (s2 <= t[3] ? s2 : t[3]), t[3] - (s1 >= t[2] ? s1 : t[2]), t[3] - t[2]);
+}
+
+// Line-integral model (lf2d_line_integ_direct / _adjoint / _sparser,
+// LongFessler2D.c:143-252, 357-470, 573-681): the trapezoid is sampled at
+// the bin centre k instead of integrated over the bin; a times its height
+// there, with the reference's expression order.
+__host__ __device__ __forceinline__ double lf_point(const LFFoot &f, int k, double a) {
+#pragma clang fp contract(off)
+  const double *t = f.t;
+  const double x = double(k);
+  if (x >= t[0] && x < t[1]) return a * (x - t[0]) / (t[1] - t[0]);
+  if (x >= t[1] && x < t[2]) return a;
+  if (x >= t[2] && x < t[3]) return a * (x - t[3]) / (t[2] - t[3]);
+  return 0.0;
 }
 
 // True when the detector coordinate of a Long & Fessler view varies faster
@@ -626,7 +643,7 @@ __device__ __forceinline__ bool lf_detector_along_y(const ViewConst &g, double R
 
 // Square tiles laid out per view and an FP64 LDS window, as in
 // fp_forward's 2-D path.
-template <typename T>
+template <typename T, bool LINE>
 __global__ __launch_bounds__(kThreads) void lf_forward(DevPlan P, const T *__restrict__ img,
                                                        T *__restrict__ sino, int views_per_block,
                                                        int lds_cap) {
@@ -673,9 +690,9 @@ __global__ __launch_bounds__(kThreads) void lf_forward(DevPlan P, const T *__res
 #pragma unroll
     for (int q = 0; q < TL::NPT; ++q) {
       if (!act[q]) continue;
-      const double a = f[q].amp * double(val[q]);
+      const double a = (LINE ? f[q].amp_line : f[q].amp) * double(val[q]);
       for (int k = f[q].kf; k <= f[q].kl; ++k) {
-        const T c = T(lf_gamma(f[q], k, P.lf_size) * a);
+        const T c = T(LINE ? lf_point(f[q], k, a) : lf_gamma(f[q], k, P.lf_size) * a);
         if (in_lds)
           atomicAdd(reg + (k - lo), double(c));
         else
@@ -693,7 +710,7 @@ __global__ __launch_bounds__(kThreads) void lf_forward(DevPlan P, const T *__res
   }
 }
 
-template <typename T>
+template <typename T, bool LINE>
 __global__ __launch_bounds__(kThreads) void lf_backward(DevPlan P, const T *__restrict__ sino,
                                                         T *__restrict__ img, int views_per_block,
                                                         int atomic_out) {
@@ -709,7 +726,12 @@ __global__ __launch_bounds__(kThreads) void lf_backward(DevPlan P, const T *__re
     LFFoot f;
     lf_footprint(P, g, tile.i, tile.j, f);
     double s = 0.0;
-    for (int k = f.kf; k <= f.kl; ++k) s += f.amp * lf_gamma(f, k, P.lf_size) * double(in[k]);
+    if (LINE) {
+      // sum += (alpha gamma) pixel[k] with gamma the trapezoid height at k
+      for (int k = f.kf; k <= f.kl; ++k) s += lf_point(f, k, f.amp_line) * double(in[k]);
+    } else {
+      for (int k = f.kf; k <= f.kl; ++k) s += f.amp * lf_gamma(f, k, P.lf_size) * double(in[k]);
+    }
     acc += T(s);
   }
   T *d = img + long(tile.j) * P.nx + tile.i;
@@ -805,7 +827,7 @@ __global__ __launch_bounds__(kThreads) void sparse_columns(DevPlan P, int *__res
     lf_footprint(P, g, i, j, f);
     for (int k = f.kf; k <= f.kl; ++k) {
       // lf_forward's coefficient for a unit value
-      const T c = T(lf_gamma(f, k, P.lf_size) * f.amp);
+      const T c = T(P.lf_line ? lf_point(f, k, f.amp_line) : lf_gamma(f, k, P.lf_size) * f.amp);
       sparse_entry(FILL, c, round6, o, n, val, row, col, view * P.nv + k, int(p));
     }
   } else {
@@ -824,6 +846,32 @@ __global__ __launch_bounds__(kThreads) void sparse_columns(DevPlan P, int *__res
     }
   }
   if (!FILL) count[q] = n;
+}
+
+// Host evaluation of one pixel's coefficient column on one view, with the
+// kernels' own footprint code: the per-pixel sparser symbols
+// (trtt_PB2D_sparser / trtt_FB2D_sparser, trtt_2D_projectors.c:393-536, on
+// the tabulated footprint; lf2d_sparser / lf2d_line_integ_sparser,
+// LongFessler2D.c:472-681).  out[bin] += coefficient for a unit value.
+// Returns false when the pixel lies in the source plane.
+bool host_column_2d(bool fan, const DevPlan &P, const ViewConst &g, int i, int j, double *out) {
+  Foot f;
+  const bool ok = fan ? voxel_footprint<FB2D, true, false>(P, g, i, j, 0, f)
+                      : voxel_footprint<PB2D, true, false>(P, g, i, j, 0, f);
+  if (!ok) return false;
+  if (f.vl < f.vr) {
+    AxisFoot<double, true, true> av;
+    av.init(P, f.vl, f.vc, f.hv, f.sv);
+    for (int m = 0; m < f.vr - f.vl; ++m) out[f.vl + m] += f.rho0 * av.next(m);
+  }
+  return true;
+}
+
+void host_lf_column(const DevPlan &P, const ViewConst &g, int i1, int i2, double *out) {
+  LFFoot f;
+  lf_footprint(P, g, i1, i2, f);
+  for (int k = f.kf; k <= f.kl; ++k)
+    out[k] += P.lf_line ? lf_point(f, k, f.amp_line) : lf_gamma(f, k, P.lf_size) * f.amp;
 }
 
 template <typename T, bool FILL>
@@ -1058,14 +1106,15 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
       // square tiles, an FP64 window of up to 2048 cells
       constexpr int S = kTile2DSquare, cap = 2048;
       const dim3 fgrid(unsigned((P.nx + S - 1) / S) * ((P.ny + S - 1) / S), grid.y);
-      if (f64)
-        hipLaunchKernelGGL(lf_forward<double>, fgrid, dim3(kThreads), cap * 8, st, P,
-                           static_cast<const double *>(in), static_cast<double *>(out),
-                           views_per_block, cap);
-      else
-        hipLaunchKernelGGL(lf_forward<float>, fgrid, dim3(kThreads), cap * 8, st, P,
-                           static_cast<const float *>(in), static_cast<float *>(out),
-                           views_per_block, cap);
+#define TRTT_LF_FWD(T, L)                                                                   \
+  hipLaunchKernelGGL((lf_forward<T, L>), fgrid, dim3(kThreads), cap * 8, st, P,              \
+                     static_cast<const T *>(in), static_cast<T *>(out), views_per_block, cap)
+      if (f64) {
+        if (P.lf_line) TRTT_LF_FWD(double, true); else TRTT_LF_FWD(double, false);
+      } else {
+        if (P.lf_line) TRTT_LF_FWD(float, true); else TRTT_LF_FWD(float, false);
+      }
+#undef TRTT_LF_FWD
     } else {
       // the views split over grid rows (atomic adds into the image) until
       // the grid offers TRTT_LF_BWD_BLOCKS workgroups: one row of 16x16
@@ -1076,14 +1125,15 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
       const int vpb = int(std::max(1L, std::min(span, (P.nviews + want - 1) / want)));
       const dim3 bgrid(unsigned(tiles), unsigned((P.nviews + vpb - 1) / vpb));
       const int atomic = (atomic_out || bgrid.y > 1) ? 1 : 0;
-      if (f64)
-        hipLaunchKernelGGL(lf_backward<double>, bgrid, dim3(kThreads), 0, st, P,
-                           static_cast<const double *>(in), static_cast<double *>(out), vpb,
-                           atomic);
-      else
-        hipLaunchKernelGGL(lf_backward<float>, bgrid, dim3(kThreads), 0, st, P,
-                           static_cast<const float *>(in), static_cast<float *>(out), vpb,
-                           atomic);
+#define TRTT_LF_BWD(T, L)                                                                   \
+  hipLaunchKernelGGL((lf_backward<T, L>), bgrid, dim3(kThreads), 0, st, P,                    \
+                     static_cast<const T *>(in), static_cast<T *>(out), vpb, atomic)
+      if (f64) {
+        if (P.lf_line) TRTT_LF_BWD(double, true); else TRTT_LF_BWD(double, false);
+      } else {
+        if (P.lf_line) TRTT_LF_BWD(float, true); else TRTT_LF_BWD(float, false);
+      }
+#undef TRTT_LF_BWD
     }
     return hipGetLastError();
   }

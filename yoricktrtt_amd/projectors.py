@@ -347,9 +347,13 @@ class Projector:
 
 
 class LFProjector(Projector):
-    """Batched Long & Fessler projector (LongFessler2D.c) over many views."""
+    """Batched Long & Fessler projector (LongFessler2D.c) over many views.
+    ``line_integ`` selects the line-integral model (the trapezoid sampled at
+    the bin centres, lf2d_line_integ_direct / _adjoint,
+    LongFessler2D.c:143-470) instead of the bin-integrated one."""
 
-    def __init__(self, params: _lib.LF2DParameters, beta, offset=None, precision: str = "f32"):
+    def __init__(self, params: _lib.LF2DParameters, beta, offset=None, precision: str = "f32",
+                 line_integ: bool = False):
         import torch
 
         self.torch = torch
@@ -366,6 +370,8 @@ class LFProjector(Projector):
                                                          _ptr(offset), beta.size)
         if not self.handle:
             raise RuntimeError("plan creation failed: " + _lib.last_error())
+        if line_integ and self.lib.trtt_hip_plan_set_lf_line(self.handle, 1) != 0:
+            raise RuntimeError(_lib.last_error())
 
     @property
     def vol_shape(self):
