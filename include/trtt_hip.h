@@ -251,6 +251,23 @@ int trtt_hip_backward(trtt_hip_plan_t *plan, const void *sino, void *vol, void *
 int trtt_hip_weighted_residual(trtt_hip_plan_t *plan, void *sino, const void *y,
                                const void *w, double *fx_dev, void *stream);
 
+/* Host-array evaluation of the weighted least-squares data term over every
+ * view of a plan, for callers that hold host arrays (the Yorick loop): the
+ * batched replacement of trtt_4D_mpy_eval (Tomography_4D/Ytrtt4D_optim.i:
+ * 87-223) and of its serial loop (:418-446).  Host FP64 arrays throughout.
+ * eval_set_data uploads the data y[view][nv][nu] and the weights w (NULL:
+ * unit weights; the reference's W_k = diag(dweights_k weight_k)) in the
+ * plan's precision and forms WCyT = sum_k A_k' W_k y_k once on the device
+ * (:764-782); call it again when the data change.  Synchronous.
+ * eval uploads x (volume series [nt][nz][ny][nx]) and returns
+ *   *fx = 1/2 sum_k (A_k x - y_k)' W_k (A_k x - y_k)   (FP64 sum)
+ *   gx  = sum_k A_k' W_k (A_k x - y_k)                  (split = 0), or
+ *       = sum_k A_k' W_k A_k x - WCyT                   (split != 0: the
+ *         reference's own split, :437-446; FP64 difference of the two terms).
+ * Synchronous; the device copies stay with the plan until it is destroyed. */
+int trtt_hip_eval_set_data(trtt_hip_plan_t *plan, const double *y, const double *w);
+int trtt_hip_eval(trtt_hip_plan_t *plan, double *fx, double *gx, const double *x, int split);
+
 /* 1 when a launch of this plan met a voxel in the source plane (the
  * reference's TRTT_FAILURE case), 0 otherwise, -1 on error.  Synchronous. */
 int trtt_hip_plan_error_flag(trtt_hip_plan_t *plan);

@@ -59,6 +59,8 @@ _SIGS = {
     "lf2d_sparser": ([C.POINTER(LF2DParameters), D, D, L, L, dp, I], None),
     "lf2d_line_integ_sparser": ([C.POINTER(LF2DParameters), D, D, L, L, dp, I], None),
     "trtt_hip_plan_set_lf_line": ([vp, I], I),
+    "trtt_hip_eval_set_data": ([vp, dp, dp], I),
+    "trtt_hip_eval": ([vp, dp, dp, dp, I], I),
     "trtt_hip_set_device": ([I], I),
     "trtt_hip_set_precision": ([I], I),
     "trtt_hip_last_error": ([], C.c_char_p),

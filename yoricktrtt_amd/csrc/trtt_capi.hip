@@ -18,7 +18,7 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
                             const void *in, void *out, const int *seg_begin, int nrows,
                             int views_per_block, int atomic_out, hipStream_t st);
 hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long n, double *fx,
-                           hipStream_t st);
+                           hipStream_t st, bool split = false);
 hipError_t launch_axial(bool cone, int w, bool f64, const DevPlan &P, int job, const void *in,
                         void *out, const int *seg_begin, int nrows, int views_per_block,
                         int atomic_out, int pmax, int pmax2, void *scratch, void *hdr,
@@ -371,9 +371,37 @@ bool finalize_plan(trtt_hip_plan *p, const double *cum) {
   return true;
 }
 
+}  // namespace
+
+// Device side of the host-array evaluation of a plan's data term.
+struct EvalData {
+  void *y = nullptr, *w = nullptr;  // data and weights, plan precision
+  void *wcyt = nullptr;             // sum_k A_k' W_k y_k (volume series)
+  void *x = nullptr, *g = nullptr;  // volume series
+  void *sino = nullptr;
+  double *stage = nullptr;          // FP64 staging of x and gx
+  double *fx = nullptr;
+};
+
+namespace {
+
+void eval_release(EvalData *e) {
+  if (!e) return;
+  hipFree(e->y);
+  hipFree(e->w);
+  hipFree(e->wcyt);
+  hipFree(e->x);
+  hipFree(e->g);
+  hipFree(e->sino);
+  hipFree(e->stage);
+  hipFree(e->fx);
+  delete e;
+}
+
 void release(trtt_hip_plan *p) {
   if (!p) return;
   hipSetDevice(p->device);
+  eval_release(p->eval);
   hipFree(p->d_views);
   hipFree(p->d_order);
   hipFree(p->d_segs);
@@ -1383,6 +1411,89 @@ int trtt_hip_dd_destroy(trtt_hip_dd_t *p) {
   return TRTT_SUCCESS;
 }
 
+// One-view plans of the drop-in symbols, cached per geometry family (kind,
+// precision, device, sizes, scales, footprint table): the unchanged Yorick
+// loop calls the same few families view after view, so a call only
+// rewrites the plan's 400-byte view constants instead of building and
+// freeing a plan.  Calls are serialised by g_mutex.
+namespace {
+
+struct DropinKey {
+  int kind, precision, device, s_deg;
+  long nx, ny, nz, nu, nv, m;
+  double s_scl, u_scl, v_scl, step, coord0;
+  unsigned long long table_hash;
+  bool operator==(const DropinKey &o) const {
+    return kind == o.kind && precision == o.precision && device == o.device &&
+           s_deg == o.s_deg && nx == o.nx && ny == o.ny && nz == o.nz && nu == o.nu &&
+           nv == o.nv && m == o.m && s_scl == o.s_scl && u_scl == o.u_scl &&
+           v_scl == o.v_scl && step == o.step && coord0 == o.coord0 &&
+           table_hash == o.table_hash;
+  }
+};
+
+// FNV-1a over the table's bytes (the key must change when the table does)
+unsigned long long hash_table(const double *t, long m) {
+  unsigned long long h = 1469598103934665603ULL;
+  const unsigned char *b = reinterpret_cast<const unsigned char *>(t);
+  for (size_t i = 0, n = size_t(m) * sizeof(double); i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+constexpr size_t kDropinCache = 8;
+struct DropinEntry {
+  DropinKey key;
+  trtt_hip_plan *plan;
+  unsigned long long used;
+};
+std::vector<DropinEntry> g_dropin;
+unsigned long long g_dropin_clock = 0;
+
+// The cached plan of `key`, its view constants set from xos / xsd.
+trtt_hip_plan *dropin_plan(const DropinKey &key, const double *cum, const double *xos,
+                           const double *xsd) {
+  for (DropinEntry &e : g_dropin)
+    if (e.key == key) {
+      trtt_hip_plan *p = e.plan;
+      e.used = ++g_dropin_clock;
+      ViewConst &g = p->h_views[0];
+      const bool is2d = p->kind == trtt::PB2D || p->kind == trtt::FB2D;
+      if (is2d)
+        view_consts_2d(g, xos, xsd, p->kind == trtt::FB2D, p->table, p->step, p->s_scl,
+                       p->s_deg, p->v_scl);
+      else
+        view_consts_3d(g, xos, xsd, p->kind == trtt::CB3D, p->s_scl, p->s_deg, p->u_scl,
+                       p->v_scl);
+      g.t0 = g.t1 = 0;
+      g.w0 = 1.0;
+      g.w1 = 0.0;
+      if (!hip_ok(hipMemcpy(p->d_views, &g, sizeof(g), hipMemcpyHostToDevice), "hipMemcpy"))
+        return nullptr;
+      axial_analysis(p);
+      if (p->axial_w > 0 && !p->d_scratch && !alloc_axial_scratch(p)) return nullptr;
+      return p;
+    }
+  trtt_hip_plan *p = make_plan(key.kind, key.precision, key.nx, key.ny, key.nz, 1, key.s_scl,
+                               key.s_deg, key.nu, key.nv, key.u_scl, key.v_scl, xos, xsd, 1, cum,
+                               key.m, key.step, key.coord0, nullptr, nullptr, nullptr, nullptr);
+  if (!p) return nullptr;
+  if (g_dropin.size() >= kDropinCache) {
+    auto lru = std::min_element(g_dropin.begin(), g_dropin.end(),
+                                [](const DropinEntry &a, const DropinEntry &b) {
+                                  return a.used < b.used;
+                                });
+    release(lru->plan);
+    g_dropin.erase(lru);
+  }
+  g_dropin.push_back(DropinEntry{key, p, ++g_dropin_clock});
+  return p;
+}
+
+}  // namespace
+
 static int dropin(int kind, double *out, double *in, long nx, long ny, long nz, double s_scl,
                   int s_deg, const double *cum, long m, double step, const double *coord,
                   long nu, long nv, double u_scl, double v_scl, const double *xos,
@@ -1392,13 +1503,13 @@ static int dropin(int kind, double *out, double *in, long nx, long ny, long nz, 
   if (!out || !in || !xos || !xsd) return fail("null argument");
   if (cum && !coord) return fail("null footprint coordinates");
   if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return TRTT_FAILURE;
-  trtt_hip_plan *p = make_plan(kind, g_precision, nx, ny, nz, 1, s_scl, s_deg, nu, nv, u_scl,
-                               v_scl, xos, xsd, 1, cum, m, step, cum ? coord[0] : 0.0, nullptr,
-                               nullptr, nullptr, nullptr);
+  const bool is2d = kind == trtt::PB2D || kind == trtt::FB2D;
+  DropinKey key{kind, g_precision, g_device, s_deg, nx, ny, is2d ? 1 : nz, is2d ? 1 : nu, nv,
+                cum ? m : 0, s_scl, u_scl, v_scl, cum ? step : 1.0, cum ? coord[0] : 0.0,
+                cum && m > 0 ? hash_table(cum, m) : 0ULL};
+  trtt_hip_plan *p = dropin_plan(key, cum, xos, xsd);
   if (!p) return TRTT_FAILURE;
-  const int r = run_dropin(p, out, in, job);
-  release(p);
-  return r;
+  return run_dropin(p, out, in, job);
 }
 
 int trtt_PB2D(double *out, double *in, long nx, long ny, double s_scl, int s_deg,
@@ -1795,6 +1906,149 @@ int trtt_hip_dropin_noalloc(int kind, double *out, const double *in, long nx, lo
   if (err) return fail("IT WAS SUPPOSED TO BE HIGHLY IMPOSSIBLE: a voxel lies in the source plane");
   if (!hip_ok(hipMemcpy(out, d_stage, sizeof(double) * nout, hipMemcpyDeviceToHost), "hipMemcpy"))
     return TRTT_FAILURE;
+  return TRTT_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Host-array evaluation of the weighted least-squares data term over every
+// view of a plan (the batched form of trtt_4D_mpy_eval,
+// Tomography_4D/Ytrtt4D_optim.i:87-223, and of its serial loop :418-446).
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+__global__ void sub_to_f64(const T *__restrict__ a, const T *__restrict__ b, double *__restrict__ c,
+                           long n, int use_b) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    c[i] = use_b ? double(a[i]) - double(b[i]) : double(a[i]);
+}
+
+template <typename T>
+__global__ void mul_into(const T *__restrict__ w, T *__restrict__ y, long n) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    y[i] *= w[i];
+}
+
+template <typename T>
+__global__ void zero_t(T *__restrict__ p, long n) {
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    p[i] = T(0);
+}
+
+// host FP64 -> device array of the plan's type
+bool upload_f64(const trtt_hip_plan *p, EvalData *e, void *dst, const double *src, long n) {
+  if (p->precision == TRTT_HIP_F64)
+    return hip_ok(hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+  if (!hip_ok(hipMemcpy(e->stage, src, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy"))
+    return false;
+  hipLaunchKernelGGL(cvt_d2f, dim3(grid_for(n)), dim3(256), 0, nullptr, e->stage,
+                     static_cast<float *>(dst), n);
+  return hip_ok(hipGetLastError(), "kernel launch");
+}
+
+template <typename T>
+void zero_dev(void *p, long n) {
+  hipLaunchKernelGGL(zero_t<T>, dim3(grid_for(n)), dim3(256), 0, nullptr, static_cast<T *>(p), n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int trtt_hip_eval_set_data(trtt_hip_plan_t *p, const double *y, const double *w) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!p || !y) return fail("null argument");
+  if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return TRTT_FAILURE;
+  const bool f64 = p->precision == TRTT_HIP_F64;
+  const size_t esz = f64 ? 8 : 4;
+  const long nsino = p->nu * p->nv * p->nviews, nvol = p->nx * p->ny * p->nz * p->nt;
+  EvalData *e = p->eval;
+  if (!e) {
+    e = new EvalData;
+    if (!hip_ok(hipMalloc(&e->y, esz * nsino), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->w, esz * nsino), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->sino, esz * nsino), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->wcyt, esz * nvol), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->x, esz * nvol), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->g, esz * nvol), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->stage, sizeof(double) * std::max(nsino, nvol)), "hipMalloc") ||
+        !hip_ok(hipMalloc(&e->fx, sizeof(double)), "hipMalloc")) {
+      eval_release(e);
+      return TRTT_FAILURE;
+    }
+    p->eval = e;
+  }
+  if (!upload_f64(p, e, e->y, y, nsino)) return TRTT_FAILURE;
+  const bool has_w = w != nullptr;
+  if (has_w) {
+    if (!upload_f64(p, e, e->w, w, nsino)) return TRTT_FAILURE;
+  } else {
+    hipFree(e->w);
+    e->w = nullptr;
+  }
+  // WCyT = sum_k A_k' W_k y_k (Ytrtt4D_optim.i:764-782)
+  hipStream_t st = nullptr;
+  if (f64) {
+    hipMemcpyAsync(e->sino, e->y, esz * nsino, hipMemcpyDeviceToDevice, st);
+    if (has_w)
+      hipLaunchKernelGGL(mul_into<double>, dim3(grid_for(nsino)), dim3(256), 0, st,
+                         static_cast<const double *>(e->w), static_cast<double *>(e->sino), nsino);
+    zero_dev<double>(e->wcyt, nvol);
+  } else {
+    hipMemcpyAsync(e->sino, e->y, esz * nsino, hipMemcpyDeviceToDevice, st);
+    if (has_w)
+      hipLaunchKernelGGL(mul_into<float>, dim3(grid_for(nsino)), dim3(256), 0, st,
+                         static_cast<const float *>(e->w), static_cast<float *>(e->sino), nsino);
+    zero_dev<float>(e->wcyt, nvol);
+  }
+  if (!hip_ok(launch_plan(p, 1, e->sino, e->wcyt, st), "backward launch")) return TRTT_FAILURE;
+  return hip_ok(hipDeviceSynchronize(), "eval set-up") ? TRTT_SUCCESS : TRTT_FAILURE;
+}
+
+int trtt_hip_eval(trtt_hip_plan_t *p, double *fx, double *gx, const double *x, int split) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  if (!p || !fx || !gx || !x) return fail("null argument");
+  EvalData *e = p->eval;
+  if (!e) return fail("trtt_hip_eval: no data (call trtt_hip_eval_set_data first)");
+  if (!hip_ok(hipSetDevice(p->device), "hipSetDevice")) return TRTT_FAILURE;
+  const bool f64 = p->precision == TRTT_HIP_F64;
+  const long nsino = p->nu * p->nv * p->nviews, nvol = p->nx * p->ny * p->nz * p->nt;
+  hipStream_t st = nullptr;
+  if (!upload_f64(p, e, e->x, x, nvol)) return TRTT_FAILURE;
+  if (f64) {
+    zero_dev<double>(e->sino, nsino);
+    zero_dev<double>(e->g, nvol);
+  } else {
+    zero_dev<float>(e->sino, nsino);
+    zero_dev<float>(e->g, nvol);
+  }
+  zero_dev<double>(e->fx, 1);
+  if (!hip_ok(hipMemsetAsync(p->d_err, 0, sizeof(int), st), "hipMemset")) return TRTT_FAILURE;
+  if (!hip_ok(launch_plan(p, 0, e->x, e->sino, st), "forward launch") ||
+      !hip_ok(trtt::launch_residual(f64, e->sino, e->y, e->w, nsino, e->fx, st, split != 0),
+              "residual launch") ||
+      !hip_ok(launch_plan(p, 1, e->sino, e->g, st), "backward launch"))
+    return TRTT_FAILURE;
+  // gx = A'W(Ax - y), or A'WAx - WCyT with split (FP64 difference)
+  if (f64)
+    hipLaunchKernelGGL(sub_to_f64<double>, dim3(grid_for(nvol)), dim3(256), 0, st,
+                       static_cast<const double *>(e->g), static_cast<const double *>(e->wcyt),
+                       e->stage, nvol, split ? 1 : 0);
+  else
+    hipLaunchKernelGGL(sub_to_f64<float>, dim3(grid_for(nvol)), dim3(256), 0, st,
+                       static_cast<const float *>(e->g), static_cast<const float *>(e->wcyt),
+                       e->stage, nvol, split ? 1 : 0);
+  if (!hip_ok(hipGetLastError(), "kernel launch")) return TRTT_FAILURE;
+  int err = 0;
+  if (!hip_ok(hipMemcpy(fx, e->fx, sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+      !hip_ok(hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy") ||
+      !hip_ok(hipMemcpy(gx, e->stage, sizeof(double) * nvol, hipMemcpyDeviceToHost), "hipMemcpy"))
+    return TRTT_FAILURE;
+  if (err) return fail("IT WAS SUPPOSED TO BE HIGHLY IMPOSSIBLE: a voxel lies in the source plane");
   return TRTT_SUCCESS;
 }
 

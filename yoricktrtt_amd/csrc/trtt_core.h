@@ -155,4 +155,7 @@ struct trtt_hip_plan {
   int axial_nbuf = 1;           // forward scratch buffers (2: reduction overlapped)
   bool two_phase_segs = false;  // some back-projection segment has t0 != t1
   trtt::AxialOverlap overlap;
+  // host-array evaluation of the data term (trtt_hip_eval*): device copies
+  // of y, W, WCyT and the work buffers, made by trtt_hip_eval_set_data
+  struct EvalData *eval = nullptr;
 };

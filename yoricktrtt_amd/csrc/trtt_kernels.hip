@@ -744,7 +744,10 @@ __global__ __launch_bounds__(kThreads) void lf_backward(DevPlan P, const T *__re
 // ---------------------------------------------------------------------------
 // Weighted residual and FP64 data-fidelity sum (Ytrtt4D_optim.i:418-446)
 // ---------------------------------------------------------------------------
-template <typename T>
+// SPLIT: s <- W s (the reference's split gradient A'W A x - WCyT,
+// Ytrtt4D_optim.i:437-446, 764-782) instead of s <- W (s - y); fx is the
+// same.
+template <typename T, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void weighted_residual(T *__restrict__ s,
                                                               const T *__restrict__ y,
                                                               const T *__restrict__ w, long n,
@@ -752,10 +755,11 @@ __global__ __launch_bounds__(kThreads) void weighted_residual(T *__restrict__ s,
   double local = 0.0;
   for (long idx = long(blockIdx.x) * kThreads + threadIdx.x; idx < n;
        idx += long(gridDim.x) * kThreads) {
-    const T d = s[idx] - y[idx];
+    const T ax = s[idx];
+    const T d = ax - y[idx];
     const T r = w ? w[idx] * d : d;
     local += double(r) * double(d);
-    s[idx] = r;
+    s[idx] = SPLIT ? (w ? w[idx] * ax : ax) : r;
   }
   __shared__ double red[kThreads / 64];
 #pragma unroll
@@ -1145,18 +1149,20 @@ hipError_t launch_projector(int kind, bool table, bool quirk, bool f64, const De
 }
 
 hipError_t launch_residual(bool f64, void *s, const void *y, const void *w, long n, double *fx,
-                           hipStream_t st) {
+                           hipStream_t st, bool split) {
   const long blocks_needed = (n + kThreads - 1) / kThreads;
   const unsigned blocks = unsigned(blocks_needed < 4096 ? (blocks_needed > 0 ? blocks_needed : 1)
                                                         : 4096);
-  if (f64)
-    hipLaunchKernelGGL(weighted_residual<double>, dim3(blocks), dim3(kThreads), 0, st,
-                       static_cast<double *>(s), static_cast<const double *>(y),
-                       static_cast<const double *>(w), n, fx);
-  else
-    hipLaunchKernelGGL(weighted_residual<float>, dim3(blocks), dim3(kThreads), 0, st,
-                       static_cast<float *>(s), static_cast<const float *>(y),
-                       static_cast<const float *>(w), n, fx);
+#define TRTT_RES(T, S)                                                                     \
+  hipLaunchKernelGGL((weighted_residual<T, S>), dim3(blocks), dim3(kThreads), 0, st,       \
+                     static_cast<T *>(s), static_cast<const T *>(y), static_cast<const T *>(w), \
+                     n, fx)
+  if (f64) {
+    if (split) TRTT_RES(double, true); else TRTT_RES(double, false);
+  } else {
+    if (split) TRTT_RES(float, true); else TRTT_RES(float, false);
+  }
+#undef TRTT_RES
   return hipGetLastError();
 }
 

@@ -305,6 +305,36 @@ class Projector:
                                                self._stream(stream)) != 0:
             raise RuntimeError(_lib.last_error())
 
+    def eval_set_data(self, y, w=None) -> None:
+        """Upload host data y (and weights w) for :meth:`eval_host`
+        (trtt_hip_eval_set_data; forms WCyT = sum_k A_k' W_k y_k)."""
+        y = _f64(y).reshape(-1)
+        if y.size != int(np.prod(self.sino_shape)):
+            raise ValueError(f"y must hold {self.sino_shape} values")
+        self._eval_keep = [y]
+        wp = None
+        if w is not None:
+            w = _f64(w).reshape(-1)
+            if w.size != y.size:
+                raise ValueError("w must have the shape of y")
+            self._eval_keep.append(w)
+            wp = _ptr(w)
+        if self.lib.trtt_hip_eval_set_data(self.handle, _ptr(y), wp) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def eval_host(self, x, split: bool = False):
+        """f(x) and g(x) of the data term on host FP64 arrays (trtt_hip_eval):
+        g = sum A'W(Ax - y), or sum A'WAx - WCyT with ``split`` (the
+        reference's form, Ytrtt4D_optim.i:437-446)."""
+        x = _f64(x)
+        if x.shape != tuple(self.vol_shape):
+            raise ValueError(f"x must have shape {self.vol_shape}")
+        g = np.zeros_like(x)
+        fx = C.c_double(0.0)
+        if self.lib.trtt_hip_eval(self.handle, C.byref(fx), _ptr(g), _ptr(x), int(split)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return fx.value, g
+
     def set_timing(self, on: bool = True) -> None:
         """Record HIP events around the main axial kernels of each call
         (not under graph capture)."""
