@@ -120,14 +120,13 @@ def algorithmic_bytes(esz=4):
     return fwd, bwd, compulsory
 
 
-def cpu_reference_baseline(geo, threads):
-    """Reference C plugin (oracle/_ref) on `threads` host threads, each
-    projecting one view forward and one view back at the full size."""
+def _reference_rate(path, geo, threads):
+    """View pairs per second of the reference C plugin built at `path` on
+    `threads` host threads, each projecting one view forward and one view
+    back at the full size (the ctypes calls release the GIL)."""
     from oracle.oracle import Reference
 
-    if not Reference.available():
-        return None
-    ref = Reference()
+    ref = Reference(path)
     rng = np.random.default_rng(1)
     vol = rng.uniform(size=(NZ, NY, NX))
     sino = rng.uniform(size=(NV, NU))
@@ -147,11 +146,127 @@ def cpu_reference_baseline(geo, threads):
     for th in ths:
         th.join()
     dt = time.perf_counter() - t0
-    view_pairs_per_s = threads / dt
-    return {"value": view_pairs_per_s / NVIEWS, "unit": "pairs/s", "cores": threads,
-            "kind": "reference",
+    return threads / dt, dt
+
+
+def cpu_reference_baseline(geo, threads):
+    """The reference C plugin (oracle/_ref, compiled from its own sources)
+    on every host core this process may use, at -O2 (the parity build) and
+    at -O3 -march=x86-64-v3; the faster build is the baseline value."""
+    from oracle.oracle import REF_O3_SO, REF_SO, Reference
+
+    if not Reference.available(REF_SO):
+        return None
+    builds = {"O2": REF_SO}
+    if Reference.available(REF_O3_SO):
+        builds["O3_x86-64-v3"] = REF_O3_SO
+    rates = {}
+    for name, path in builds.items():
+        vps, dt = _reference_rate(path, geo, threads)
+        rates[name] = {"pairs_per_s": vps / NVIEWS, "wall_s": round(dt, 2)}
+    best = max(rates, key=lambda k: rates[k]["pairs_per_s"])
+    return {"value": rates[best]["pairs_per_s"], "unit": "pairs/s", "cores": threads,
+            "kind": "reference", "nproc": os.cpu_count(), "build": best, "builds": rates,
             "sample": f"{threads} threads x (1 forward + 1 back view of trtt_CB3D_cubic, FP64, "
-                      f"256^3 -> 512x384), {dt:.1f} s wall, scaled to 360 views"}
+                      f"256^3 -> 512x384) per build, scaled to 360 views"}
+
+
+def error_vs_fp64(geo, x_host, y_host, sino, vol, views=(0, 90, 180, 270), slab=(124, 132)):
+    """Max relative error of the FP32 GPU results of the last timed step
+    against the FP64 oracle: the forward on a few views, the back
+    projection of all 360 views on a slab of slices."""
+    from oracle.oracle import Oracle
+
+    o = Oracle()
+    views = np.asarray(views)
+    rf = o.views_fwd3d(x_host.astype(np.float64), NX, NY, NZ, 1.0, NU, NV, 1.0, 1.0,
+                       geo.xos[views], geo.xsd[views], True)
+    got_f = sino[views].double().cpu().numpy()
+    rb = o.views_bwd3d(y_host.astype(np.float64), NX, NY, NZ, 1.0, NU, NV, 1.0, 1.0, geo.xos,
+                       geo.xsd, True, k0=slab[0], k1=slab[1])
+    got_b = vol[slab[0]:slab[1]].double().cpu().numpy()
+    ef = float(max(np.abs(got_f[q] - rf[q]).max() / np.abs(rf[q]).max() for q in range(views.size)))
+    eb = float(np.abs(got_b - rb).max() / np.abs(rb).max())
+    return {"forward_max_rel": float(f"{ef:.3g}"), "backward_max_rel": float(f"{eb:.3g}"),
+            "measure": "max|gpu - fp64| / max|fp64|",
+            "sample": f"forward views {list(map(int, views))}; back projection of all "
+                      f"{NVIEWS} views on slices {slab[0]}..{slab[1] - 1}"}
+
+
+def table_model_rate(x, y, steps=3):
+    """Throughput of the tabulated footprint (the model the Yorick scripts
+    call, Ytrtt_3D_projectors.i:124-128; 10^5-entry table as
+    Ytrtt3D_script_reconstructions.i builds it) on the same workload."""
+    import torch
+
+    from yoricktrtt_amd.projectors import Projector
+
+    P = Projector(geometry(), model="table", precision="f32", size_footprint=100000)
+    sino = torch.zeros((NVIEWS, NV, NU), device=x.device, dtype=torch.float32)
+    vol = torch.zeros_like(x)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    P.forward(x, sino)
+    P.backward(y, vol)
+    fw, bw = [], []
+    for _ in range(steps):
+        sino.zero_()
+        vol.zero_()
+        ev[0].record()
+        P.forward(x, sino)
+        ev[1].record()
+        P.backward(y, vol)
+        ev[2].record()
+        torch.cuda.synchronize()
+        fw.append(ev[0].elapsed_time(ev[1]))
+        bw.append(ev[1].elapsed_time(ev[2]))
+    P.close()
+    f, b = float(np.mean(fw)), float(np.mean(bw))
+    return {"pairs_per_s": round(1e3 / (f + b), 3), "forward_ms": round(f, 2),
+            "backward_ms": round(b, 2), "kernels": "general footprint kernels (FP64 table "
+            "coordinates, FP32 data)", "steps": steps}
+
+
+def pcie_rates(P, x_host, y_host, views=4):
+    """Rates with host arrays, PCIe transfers included (not the bench value):
+    the per-view drop-in symbol trtt_CB3D_cubic as the unchanged Yorick loop
+    calls it (FP64 host volume up and projection down per view), and the
+    batched host-array evaluation trtt_hip_eval of the whole data term."""
+    import ctypes as C
+
+    from yoricktrtt_amd import _lib
+
+    lib = _lib.load()
+    geo = P.geom
+    dp = _lib.dp
+    x64 = np.ascontiguousarray(x_host, dtype=np.float64)
+    y0 = np.ascontiguousarray(y_host[0], dtype=np.float64)
+    out_f = np.zeros((NV, NU))
+    out_b = np.zeros((NZ, NY, NX))
+
+    def pair(v):
+        xos, xsd = np.ascontiguousarray(geo.xos[v]), np.ascontiguousarray(geo.xsd[v])
+        for out, inp, job in ((out_f, x64, 0), (out_b, y0, 1)):
+            if lib.trtt_CB3D_cubic(out.ctypes.data_as(dp), inp.ctypes.data_as(dp), NX, NY, NZ, 1.0,
+                                   3, NU, NV, 1.0, 1.0, xos.ctypes.data_as(dp),
+                                   xsd.ctypes.data_as(dp), job) != 0:
+                raise RuntimeError(_lib.last_error())
+
+    pair(0)  # builds the cached one-view plan
+    t0 = time.perf_counter()
+    for v in range(1, views + 1):
+        pair(v * (NVIEWS // (views + 1)))
+    per_view_pair = (time.perf_counter() - t0) / views
+    P.eval_set_data(y_host)
+    P.eval_host(x64)
+    t0 = time.perf_counter()
+    reps = 2
+    for _ in range(reps):
+        P.eval_host(x64)
+    t_eval = (time.perf_counter() - t0) / reps
+    return {"dropin_pairs_per_s": round(1.0 / (per_view_pair * NVIEWS), 4),
+            "dropin_sample": f"{views} views, forward + back each, FP64 host arrays, FP32 compute",
+            "eval_host_pairs_per_s": round(1.0 / t_eval, 3),
+            "eval_host_sample": f"{reps} evaluations of trtt_hip_eval (x up, gx down, FP64 host)"}
 
 
 def main():
@@ -161,7 +276,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--model", default="cubic", choices=["cubic", "table"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("BENCH_CPU_THREADS", 16)))
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("BENCH_CPU_THREADS", len(os.sched_getaffinity(0)))))
+    ap.add_argument("--no-check", action="store_true", help="skip the error statement vs FP64")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the table-model and host-array (PCIe) figures")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run an elementwise copy of 1 GiB (HBM counter calibration)")
@@ -298,6 +417,17 @@ def main():
     result["roofline"]["traffic"] = measured_traffic(tuning, result["roofline"]["kernel"])
     # the axial kernels are bound by instruction issue, not by HBM
     result["roofline"]["issue"] = measured_issue(tuning, result["roofline"]["kernel"], dom_ms)
+    x_host = x.cpu().numpy()
+    y_host = y.cpu().numpy()
+    if not args.no_check:
+        print("bench: error against the FP64 oracle", file=sys.stderr, flush=True)
+        result["error_vs_fp64"] = error_vs_fp64(geo, x_host, y_host, sino, vol)
+    if not args.no_extra and args.model == "cubic":
+        print("bench: table model", file=sys.stderr, flush=True)
+        result["table_model"] = table_model_rate(x, y)
+        print("bench: host-array rates", file=sys.stderr, flush=True)
+        P.set_timing(False)
+        result["pcie"] = pcie_rates(P, x_host, y_host)
     if not args.no_cpu:
         print("bench: timing the reference C plugin on the host", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_reference_baseline(geo, args.cpu_threads)

@@ -17,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libtrtt_ref.so")
+REF_O3_SO = os.path.join(HERE, "_ref", "libtrtt_ref_o3.so")
 
 dp = C.POINTER(C.c_double)
 L, I, D = C.c_long, C.c_int, C.c_double
@@ -249,8 +250,8 @@ class Reference:
         self.lib = lib
 
     @staticmethod
-    def available() -> bool:
-        return os.path.exists(REF_SO)
+    def available(path: str = REF_SO) -> bool:
+        return os.path.exists(path)
 
     def CB3D_cubic(self, out, inp, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, job, deg=3):
         return self.lib.trtt_CB3D_cubic(_p(out), _p(_f64(inp)), nx, ny, nz, s, deg, nu, nv, us, vs,
