@@ -202,6 +202,11 @@ def table_model_rate(x, y, steps=3):
     from yoricktrtt_amd.projectors import Projector
 
     P = Projector(geometry(), model="table", precision="f32", size_footprint=100000)
+    if P.path > 0:
+        from yoricktrtt_amd.autotune import autotune
+
+        autotune(P, cache=os.path.join(HERE, "yoricktrtt_amd", "tuning", "tuning.json"))
+    path = P.path
     sino = torch.zeros((NVIEWS, NV, NU), device=x.device, dtype=torch.float32)
     vol = torch.zeros_like(x)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -221,9 +226,10 @@ def table_model_rate(x, y, steps=3):
         bw.append(ev[1].elapsed_time(ev[2]))
     P.close()
     f, b = float(np.mean(fw)), float(np.mean(bw))
+    kern = ("axial kernels, FP64 table indices, cubic CDF at the table nodes" if path > 0
+            else "general footprint kernels (FP64 table coordinates, FP32 data)")
     return {"pairs_per_s": round(1e3 / (f + b), 3), "forward_ms": round(f, 2),
-            "backward_ms": round(b, 2), "kernels": "general footprint kernels (FP64 table "
-            "coordinates, FP32 data)", "steps": steps}
+            "backward_ms": round(b, 2), "kernels": kern, "steps": steps}
 
 
 def pcie_rates(P, x_host, y_host, views=4):

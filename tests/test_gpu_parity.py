@@ -263,7 +263,8 @@ def test_batched_plan_vs_oracle(lib, oracle, kind, model, prec):
     vol = rng.uniform(size=(nz, ny, nx))
     sino = rng.uniform(size=(V, nv, nu))
     P = Projector(geo, model=model, precision=prec, size_footprint=10000)
-    expect_axial = model == "cubic" and prec == "f32"
+    # FP32 plans take the axial kernels, with a cubic CDF table too
+    expect_axial = prec == "f32"
     assert (P.path > 0) == expect_axial, P.path
     dt = torch.float64 if prec == "f64" else torch.float32
     fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
@@ -474,7 +475,18 @@ def test_tilted_orbit_uses_general_kernels(lib, oracle, kind):
     P.close()
 
 
-def test_axial_wide_footprints_and_truncation(lib, oracle):
+def _tab(model, m=10000):
+    """(cum, coord, step) of an m-entry cubic table for the oracle, or None."""
+    if model != "table":
+        return None
+    from yoricktrtt_amd.geometry import footprint_table
+
+    _, cum, coord, step = footprint_table(m)
+    return cum, coord, step
+
+
+@pytest.mark.parametrize("model", ["cubic", "table"])
+def test_axial_wide_footprints_and_truncation(lib, oracle, model):
     """Large magnification (W = 13 kernels) and a detector smaller than the
     projected volume (windows clipped at every edge, hulls at the borders)."""
     from yoricktrtt_amd.geometry import cone_geometry
@@ -485,7 +497,7 @@ def test_axial_wide_footprints_and_truncation(lib, oracle):
     us, vs = 1.0, 1.0
     geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, us, vs, np.linspace(0.0, 6.0, V), 70.0, 120.0,
                         u_off=3.0, v_off=-2.0)
-    P = Projector(geo, model="cubic", precision="f32")
+    P = Projector(geo, model=model, precision="f32")
     assert P.path == 13
     rng = np.random.default_rng(8)
     vol = rng.uniform(size=(nz, ny, nx))
@@ -495,16 +507,18 @@ def test_axial_wide_footprints_and_truncation(lib, oracle):
     rb = np.zeros((nz, ny, nx))
     for v in range(V):
         rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v],
-                         geo.xsd[v], True, 0)
+                         geo.xsd[v], True, 0, _tab(model))
         assert rel_err(fwd[v], rf) < 1e-5
-        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v], geo.xsd[v], True, 1)
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v], geo.xsd[v], True, 1,
+                    _tab(model))
     assert rel_err(bwd, rb) < 1e-5
     P.close()
 
 
 @pytest.mark.parametrize("kind,us", [("parallel", 1.5), ("parallel", 2.5), ("parallel", 3.5),
                                      ("parallel", 4.5), ("cone", 2.5), ("cone", 5.0)])
-def test_axial_coarse_detector_vs_oracle(lib, oracle, kind, us):
+@pytest.mark.parametrize("model", ["cubic", "table"])
+def test_axial_coarse_detector_vs_oracle(lib, oracle, kind, us, model):
     """Detector pixels along the rotation axis coarser than the projected
     slice pitch: voxels of neighbouring slices start their windows on the
     same pixel, so the forward accumulates its column profiles in 2, 3 or 4
@@ -522,7 +536,7 @@ def test_axial_coarse_detector_vs_oracle(lib, oracle, kind, us):
     else:
         nu = int(math.ceil(64 / us)) + 6
         geo = parallel_geometry_3d(nx, ny, nz, 1.0, nu, nv, us, 1.0, th)
-    P = Projector(geo, model="cubic", precision="f32")
+    P = Projector(geo, model=model, precision="f32")
     assert P.path in (9, 13)
     rng = np.random.default_rng(int(us * 10))
     vol = rng.uniform(size=(nz, ny, nx))
@@ -532,16 +546,17 @@ def test_axial_coarse_detector_vs_oracle(lib, oracle, kind, us):
     rb = np.zeros((nz, ny, nx))
     for v in range(V):
         rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, 1.0, geo.xos[v],
-                         geo.xsd[v], kind == "cone", 0)
+                         geo.xsd[v], kind == "cone", 0, _tab(model))
         assert rel_err(fwd[v], rf) < 1e-5, v
         oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, 1.0, geo.xos[v], geo.xsd[v],
-                    kind == "cone", 1)
+                    kind == "cone", 1, _tab(model))
     assert rel_err(bwd, rb) < 1e-5
     P.close()
 
 
 @pytest.mark.parametrize("nt", [1, 3])
-def test_axial_backward_tiles_vs_oracle(lib, oracle, nt):
+@pytest.mark.parametrize("model", ["cubic", "table"])
+def test_axial_backward_tiles_vs_oracle(lib, oracle, nt, model):
     """Every backward launch shape, including the tiles of 128 slices (two
     voxels per lane), against the FP64 oracle: a slice count that leaves a
     partial last tile for both k extents, and (nt = 3) views interpolated
@@ -569,13 +584,13 @@ def test_axial_backward_tiles_vs_oracle(lib, oracle, nt):
         w1[t0 == t1] = 0.0
         w0 = 1.0 - w1
         kw = dict(nt=nt, t0=t0, t1=t1, w0=w0, w1=w1)
-    P = Projector(geo, model="cubic", precision="f32", **kw)
+    P = Projector(geo, model=model, precision="f32", **kw)
     assert P.path > 0
     y = rng.uniform(size=(V, nv, nu))
     rb = np.zeros((nt, nz, ny, nx))
     for v in range(V):
         bv = oracle.sb3d(np.zeros((nz, ny, nx)), y[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0,
-                         geo.xos[v], geo.xsd[v], True, 1)
+                         geo.xos[v], geo.xsd[v], True, 1, _tab(model))
         rb[t0[v]] += w0[v] * bv
         rb[t1[v]] += w1[v] * bv
     for tile in sorted(AT.BWD_TILES):
@@ -586,7 +601,8 @@ def test_axial_backward_tiles_vs_oracle(lib, oracle, nt):
 
 
 
-def test_axial_direct_path_and_edges_vs_oracle(lib, oracle):
+@pytest.mark.parametrize("model", ["cubic", "table"])
+def test_axial_direct_path_and_edges_vs_oracle(lib, oracle, model):
     """The smallest LDS windows send most tiles down the direct path of both
     kernels (the block entered when a lane's window does not fit), on a
     detector smaller than the projected volume, so that windows cross both
@@ -599,20 +615,86 @@ def test_axial_direct_path_and_edges_vs_oracle(lib, oracle):
     nx, ny, nz, nu, nv, V = 26, 23, 60, 52, 44, 5
     geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, np.linspace(0.3, 5.9, V), 110.0, 170.0,
                         u_off=-2.5, v_off=1.5)
-    P = Projector(geo, model="cubic", precision="f32")
+    P = Projector(geo, model=model, precision="f32")
     assert P.path > 0
     rng = np.random.default_rng(17)
     vol = rng.uniform(size=(nz, ny, nx))
     sino = rng.uniform(size=(V, nv, nu))
     rf = np.stack([oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0,
-                               geo.xos[v], geo.xsd[v], True, 0) for v in range(V)])
+                               geo.xos[v], geo.xsd[v], True, 0, _tab(model)) for v in range(V)])
     rb = np.zeros((nz, ny, nx))
     for v in range(V):
-        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v], True, 1)
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v], True, 1,
+                    _tab(model))
     for wf, wb in [(4096, 4096), (8192, 16384)]:
         AT.set_tuning(P, fwd_window_bytes=wf, bwd_window_bytes=wb)
         fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
         bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
         assert rel_err(fwd, rf) < 1e-5, wf
         assert rel_err(bwd, rb) < 1e-5, wb
+    P.close()
+
+
+@pytest.mark.parametrize("case", ["quadratic", "perturbed", "short"])
+def test_table_not_cubic_cdf_uses_general_kernels(lib, oracle, case):
+    """Only a table that holds the cubic B-spline CDF takes the axial
+    kernels: a degree-2 footprint (s_deg = 2, as the reference's 4-D
+    benchmark script builds it), a cubic table with a perturbation of 1e-4,
+    and a 500-entry cubic table (trapezoid error 1.4e-5) run the general
+    kernels, and all match the oracle with their own table."""
+    from yoricktrtt_amd.geometry import cone_geometry, footprint_table
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("table fallback", case)
+    nx, ny, nz, nu, nv, V = 22, 20, 18, 30, 34, 4
+    deg = 2 if case == "quadratic" else 3
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, np.linspace(0.1, 4.0, V), 80.0, 130.0,
+                        s_deg=deg)
+    _, cum, coord, step = footprint_table(500 if case == "short" else 10000, deg)
+    if case == "perturbed":
+        cum = cum + 1e-4 * np.sin(np.arange(cum.size) * 0.01)
+    P = Projector(geo, model="table", precision="f32", table=(cum, coord, step))
+    assert P.path == 0, case
+    rng = np.random.default_rng(21)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v],
+                         geo.xsd[v], True, 0, (cum, coord, step), deg)
+        assert rel_err(fwd[v], rf) < 1e-5
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v], True, 1,
+                    (cum, coord, step), deg)
+    assert rel_err(bwd, rb) < 1e-5
+    P.close()
+
+
+@pytest.mark.parametrize("m", [10000, 100000])
+def test_axial_table_cone_vs_oracle(lib, oracle, m):
+    """The axial table path on a cone plan with the 10^4 (2-D / 4-D scripts)
+    and 10^5 (3-D script) tables, every view forward and back."""
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("axial table", m)
+    nx, ny, nz, nu, nv, V = 40, 36, 70, 64, 72, 9
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, np.linspace(0.0, 6.0, V), 120.0, 180.0)
+    P = Projector(geo, model="table", precision="f32", size_footprint=m)
+    assert P.path > 0
+    rng = np.random.default_rng(m % 97)
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+    bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+    tab = _tab("table", m)
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v],
+                         geo.xsd[v], True, 0, tab)
+        assert rel_err(fwd[v], rf) < 1e-5, v
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v], True, 1,
+                    tab)
+    assert rel_err(bwd, rb) < 1e-5
     P.close()

@@ -94,10 +94,27 @@ struct Col {
   T sv;            // 1 / hv
   int vl, vr;      // v window
   bool ok;
+  // tabulated footprint: FP64 table coordinate of the window start and
+  // its step per pixel along v; hu / du and 1 / (a4^2 + vca8^2) in FP64
+  double xv0, sdv, hu0d, inv_ad;
 };
 
-template <bool CONE, typename T>
+// Tabulated footprint (trtt_PB3D / trtt_CB3D with a table of the cubic
+// B-spline CDF, trtt_3D_projectors.c:267-300): the weight of a pixel is the
+// difference of the table at the floor() indices of its two bounds, and the
+// table at index n holds the cubic CDF at node coord0 + n step (to the
+// table's own trapezoid error; the host accepts a table for this path only
+// when it is within 1e-6 of that).  The index is taken in FP64 from the
+// reference's coordinate expressions, the CDF at the node in T.
+template <typename T>
+__device__ __forceinline__ T table_cdf(const DevPlan &P, double x) {
+  const double c = fmin(fmax(x, 0.0), double(P.m - 1));
+  return cdf3c<T>(T(fma(floor(c), P.step, P.coord0)));
+}
+
+template <bool CONE, bool TAB, typename T>
 __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int i, int j, Col<T> &c) {
+#pragma clang fp contract(off)
   const double di = double(i), dj = double(j);
   if (CONE) {
     const double wk = (g.o4 + g.o2 * dj) + g.o1 * di;
@@ -120,6 +137,14 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.inv_a = T(1.0 / a);
     clampw(c.vc, P.hs0 * hv, P.nv, c.vl, c.vr);
     c.vcf = T(c.vc - double(c.vl));
+    if (TAB) {
+      // trtt_3D_projectors.c:267-275: scl_norm_v / step and the start
+      const double sdv = (1.0 / hv) / P.step;
+      c.sdv = sdv;
+      c.xv0 = ((double(c.vl) - c.vc) - hv * P.coord0) * sdv;
+      c.hu0d = P.hu_init * magn;
+      c.inv_ad = 1.0 / a;
+    }
   } else {
     c.ok = true;
     c.vc = (g.nvc + g.nvy * dj) + g.nvx * di;
@@ -131,6 +156,13 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.inv_a = T(0);
     clampw(c.vc, g.pb_hsv, P.nv, c.vl, c.vr);
     c.vcf = T(c.vc - double(c.vl));
+    if (TAB) {
+      const double sdv = g.pb_sv / P.step;
+      c.sdv = sdv;
+      c.xv0 = ((double(c.vl) - c.vc) - g.pb_hv * P.coord0) * sdv;
+      c.hu0d = g.pb_hu;
+      c.inv_ad = 0.0;
+    }
   }
 }
 
@@ -145,6 +177,7 @@ struct Vox {
   int ul;        // first pixel, before clamping at the left detector edge
   int ur;        // end pixel, clamped at the right detector edge
   int ux;        // end pixel before clamping
+  double xu0, sdu;  // tabulated footprint: table coordinate at ul, step per pixel
 };
 
 // Pixel window of the support [c - hs, c + hs] on a detector of n pixels:
@@ -164,7 +197,7 @@ __device__ __forceinline__ bool on_detector(const Vox<T> &v) {
   return v.ur > max(v.ul, 0);
 }
 
-template <typename T>
+template <bool TAB = false, typename T>
 __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T> &c, int k,
                                       Vox<T> &v) {
   v.uc = c.uc0 + c.duc * double(k);
@@ -175,6 +208,24 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
   v.rho = c.rho * du;
   v.hs = float(T(P.hs0) * hu);
   clampw_f(float(v.uc), v.hs, P.nu, v.ul, v.ur, v.ux);
+  if (TAB) {
+    // FP64 scale and table coordinate (trtt_3D_projectors.c:1264-1280,
+    // 267-275): hu = hu_init magn du, x = ((ul - uc) - hu coord0) / (hu step);
+    // the square root and reciprocal start from single precision and take
+    // one Newton step each (relative error far below the table's index
+    // resolution)
+    const double ud = (v.uc - a12) * P.u_scl;
+    const double dsq = fma(ud * ud, c.inv_ad, 1.0);
+    double dud = double(fsqrt(float(dsq)));
+    dud = 0.5 * (dud + dsq / dud);
+    const double hud = c.hu0d * dud;
+    const double den = hud * P.step;
+    double r = double(frcp(float(den)));
+    r = fma(r, fma(-den, r, 1.0), r);
+    r = fma(r, fma(-den, r, 1.0), r);
+    v.sdu = r;
+    v.xu0 = ((double(v.ul) - v.uc) - hud * P.coord0) * r;
+  }
 }
 
 // Weights of pixels [ul, ul+NP) for a voxel whose unclamped window
@@ -182,13 +233,14 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
 // weight is 1/2 - F because the CDF is exactly 1/2 from the footprint's right
 // end on.  The window starts (unclamped) at least 2h left of the centre,
 // where the CDF is exactly -1/2.  Entries past the window are zero.
-template <int NP, typename T>
-__device__ __forceinline__ void uweights(const Vox<T> &v, T (&w)[NP]) {
+template <int NP, bool TAB = false, typename T>
+__device__ __forceinline__ void uweights(const DevPlan &P, const Vox<T> &v, T (&w)[NP]) {
   const T x0 = T(double(v.ul) - v.uc) * v.su;
   T prev = T(-0.5);
 #pragma unroll
   for (int l = 0; l + 1 < NP; ++l) {
-    const T nx = cdf3c<T>(x0 + T(l + 1) * v.su);
+    const T nx = TAB ? table_cdf<T>(P, fma(double(l + 1), v.sdu, v.xu0))
+                     : cdf3c<T>(x0 + T(l + 1) * v.su);
     w[l] = nx - prev;
     prev = nx;
   }
@@ -223,9 +275,10 @@ __device__ __forceinline__ float from_next_lane(float v) {
 // v weights of the column: lane m evaluates the CDF at the m-th v pixel
 // bound, one shifted difference gives the weight of pixel m in lane m (zero
 // from the window's end on), and readlane broadcasts the W weights.
-template <int W, typename T>
-__device__ __forceinline__ void vweights(const Col<T> &c, int lane, T (&cv)[W]) {
-  const T f = cdf3c<T>((T(lane) - c.vcf) * c.sv);
+template <int W, bool TAB = false, typename T>
+__device__ __forceinline__ void vweights(const DevPlan &P, const Col<T> &c, int lane, T (&cv)[W]) {
+  const T f = TAB ? table_cdf<T>(P, fma(double(lane), c.sdv, c.xv0))
+                  : cdf3c<T>((T(lane) - c.vcf) * c.sv);
   // the shift runs on all lanes (a lane reading a disabled lane gets 0)
   const T fn = from_next_lane(f);
   const T d = lane < c.vr - c.vl ? fn - f : T(0);
@@ -397,10 +450,11 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
 // m > 1 where voxels m slices apart do, so the lanes of one residue of k
 // mod m run the plain steps together, one residue after the other; 0:
 // LDS atomics.
-template <int NP, typename T>
-__device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, int passes) {
+template <int NP, bool TAB, typename T>
+__device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v, T *pp, T val,
+                                              int passes) {
   T w[NP];
-  uweights<NP>(v, w);
+  uweights<NP, TAB>(P, v, w);
   if (passes == 1) {
 #pragma unroll
     for (int l = 0; l < NP; ++l) {
@@ -428,10 +482,10 @@ __device__ __forceinline__ void profile_steps(const Vox<T> &v, T *pp, T val, int
 // Backward gather of one voxel whose window has at most NP pixels against
 // the column's Q row (qq: Q at the window's first pixel; Q is zero past the
 // column span).
-template <int NP, typename T>
-__device__ __forceinline__ T gather_steps(const Vox<T> &v, const T *qq) {
+template <int NP, bool TAB, typename T>
+__device__ __forceinline__ T gather_steps(const DevPlan &P, const Vox<T> &v, const T *qq) {
   T w[NP];
-  uweights<NP>(v, w);
+  uweights<NP, TAB>(P, v, w);
   T sum = T(0);
 #pragma unroll
   for (int l = 0; l < NP; ++l) sum += w[l] * qq[l];
@@ -439,12 +493,13 @@ __device__ __forceinline__ T gather_steps(const Vox<T> &v, const T *qq) {
 }
 
 // Dispatch on the column's wave-uniform pixel bound np (1..W).
-template <int W, typename T>
-__device__ __forceinline__ void profile_n(int np, const Vox<T> &v, T *pp, T val, int pl) {
+template <int W, bool TAB, typename T>
+__device__ __forceinline__ void profile_n(const DevPlan &P, int np, const Vox<T> &v, T *pp, T val,
+                                          int pl) {
   switch (np) {
 #define TRTT_PS(NN) \
   case NN:          \
-    if (NN <= W) profile_steps<(NN <= W ? NN : 1), T>(v, pp, val, pl); \
+    if (NN <= W) profile_steps<(NN <= W ? NN : 1), TAB, T>(P, v, pp, val, pl); \
     break;
     TRTT_PS(1) TRTT_PS(2) TRTT_PS(3) TRTT_PS(4) TRTT_PS(5) TRTT_PS(6) TRTT_PS(7) TRTT_PS(8)
     TRTT_PS(9) TRTT_PS(10) TRTT_PS(11) TRTT_PS(12) TRTT_PS(13)
@@ -453,12 +508,12 @@ __device__ __forceinline__ void profile_n(int np, const Vox<T> &v, T *pp, T val,
   }
 }
 
-template <int W, typename T>
-__device__ __forceinline__ T gather_n(int np, const Vox<T> &v, const T *qq) {
+template <int W, bool TAB, typename T>
+__device__ __forceinline__ T gather_n(const DevPlan &P, int np, const Vox<T> &v, const T *qq) {
   switch (np) {
 #define TRTT_GS(NN) \
   case NN:          \
-    return NN <= W ? gather_steps<(NN <= W ? NN : 1), T>(v, qq) : T(0);
+    return NN <= W ? gather_steps<(NN <= W ? NN : 1), TAB, T>(P, v, qq) : T(0);
     TRTT_GS(1) TRTT_GS(2) TRTT_GS(3) TRTT_GS(4) TRTT_GS(5) TRTT_GS(6) TRTT_GS(7) TRTT_GS(8)
     TRTT_GS(9) TRTT_GS(10) TRTT_GS(11) TRTT_GS(12) TRTT_GS(13)
 #undef TRTT_GS
@@ -467,7 +522,7 @@ __device__ __forceinline__ T gather_n(int np, const Vox<T> &v, const T *qq) {
 }
 
 // Slow path for one voxel: arbitrary window sizes, direct global memory.
-template <bool FWD, typename T>
+template <bool FWD, bool TAB, typename T>
 __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T val, T *out,
                            const T *in, T &acc) {
   const int ul = max(v.ul, 0);  // the part of the window on the detector
@@ -475,15 +530,23 @@ __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T
   if (npu <= 0 || npv <= 0) return;
   const T xu0 = T(double(ul) - v.uc) * v.su;
   const T xv0 = -c.vcf * c.sv;
-  T fv0 = cdf3c<T>(xv0);
+  // tabulated footprint: u coordinates from the window start ul (>= v.ul)
+  auto fu = [&](int l) {
+    return TAB ? table_cdf<T>(P, fma(double(ul - v.ul + l), v.sdu, v.xu0))
+               : cdf3c<T>(xu0 + T(l) * v.su);
+  };
+  auto fv = [&](int m) {
+    return TAB ? table_cdf<T>(P, fma(double(m), c.sdv, c.xv0)) : cdf3c<T>(xv0 + T(m) * c.sv);
+  };
+  T fv0 = fv(0);
   T sum = T(0);
   for (int m = 0; m < npv; ++m) {
-    const T fv1 = cdf3c<T>(xv0 + T(m + 1) * c.sv);
+    const T fv1 = fv(m + 1);
     const T wv = fv1 - fv0;
     fv0 = fv1;
-    T fu0 = cdf3c<T>(xu0);
+    T fu0 = fu(0);
     for (int l = 0; l < npu; ++l) {
-      const T fu1 = cdf3c<T>(xu0 + T(l + 1) * v.su);
+      const T fu1 = fu(l + 1);
       const T w = wv * (fu1 - fu0);
       fu0 = fu1;
       const long pix = long(c.vl + m) * P.nu + (ul + l);
@@ -563,7 +626,7 @@ struct ColLane {
 };
 
 // Column `col` of the tile on view g; `active` false leaves the lane empty.
-template <bool CONE, int CX, int CY, typename T>
+template <bool CONE, bool TAB, int CX, int CY, typename T>
 __device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &g, const TileMap &tm,
                                              int col, bool active, int kf, int kl, ColLane<T> &cl) {
   cl.i = tm.bx * CX + col % CX;
@@ -574,7 +637,7 @@ __device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &
   cl.lo = cl.hi = 0;
   cl.np = 0;
   if (active && cl.i < P.nx && cl.j < P.ny) {
-    column<CONE>(P, g, cl.i, cl.j, cl.c);
+    column<CONE, TAB>(P, g, cl.i, cl.j, cl.c);
     cl.bad = !cl.c.ok;
     if (cl.c.ok && cl.c.vl < cl.c.vr) {
       column_hull(P, g, cl.c, kf, kl, cl.lo, cl.hi, cl.np);
@@ -583,17 +646,23 @@ __device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &
   }
 }
 
-template <bool CONE, int CX, int CY, int NCOL, typename T>
+template <bool CONE, bool TAB, int CX, int CY, int NCOL, typename T>
 __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &g, const TileMap &tm,
                                              int wave, int lane, int kf, int kl, ColLane<T> &cl) {
-  column_entry<CONE, CX, CY>(P, g, tm, wave * NCOL + lane, lane < NCOL, kf, kl, cl);
+  column_entry<CONE, TAB, CX, CY>(P, g, tm, wave * NCOL + lane, lane < NCOL, kf, kl, cl);
 }
 
 __device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-template <typename T>
+template <bool TAB = false, typename T>
 __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> &c, int &lo, int &hi,
                                           int &np, int &i, int &j) {
+  if (TAB) {
+    c.xv0 = bcast(src.c.xv0, cc);
+    c.sdv = bcast(src.c.sdv, cc);
+    c.hu0d = bcast(src.c.hu0d, cc);
+    c.inv_ad = bcast(src.c.inv_ad, cc);
+  }
   c.vcf = bcast(src.c.vcf, cc);
   c.uc0 = bcast(src.c.uc0, cc);
   c.duc = bcast(src.c.duc, cc);
@@ -668,7 +737,7 @@ __device__ __forceinline__ void block_window(const ColLane<T> &cl, bool live, in
 #define TRTT_BWD_OCC
 #endif
 
-template <bool CONE, int W, int CX, int CY, typename T>
+template <bool CONE, int W, int CX, int CY, bool TAB, typename T>
 __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__restrict__ vol,
                                           T *__restrict__ sino, int views_per_block, int wcap,
                                           int pmax, int view_base, int view_end,
@@ -702,8 +771,8 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
   if (multi) {
     const int slot = lane / NCOL;
     const bool active = slot < views_per_block && first_view + slot < view_end;
-    column_entry<CONE, CX, CY>(P, P.views[active ? first_view + slot : first_view], tm,
-                               lane % NCOL, active, kf, kl, cl);
+    column_entry<CONE, TAB, CX, CY>(P, P.views[active ? first_view + slot : first_view], tm,
+                                    lane % NCOL, active, kf, kl, cl);
     if (cl.bad) atomicOr(P.err, 1);
   }
 
@@ -722,7 +791,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     const T *vol1 = vol + long(g.t1) * P.vox_per_phase;
 
     if (!multi) {
-      column_table<CONE, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
+      column_table<CONE, TAB, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
       if (cl.bad) atomicOr(P.err, 1);
     }
     // this view's columns sit in lanes loff .. loff + NCOL - 1
@@ -771,17 +840,17 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         if (pend) load_col(__builtin_ctzll(pend));
         Col<T> c;
         int lo, hi, np, i, j;
-        bcast_col(cl, cc, c, lo, hi, np, i, j);
+        bcast_col<TAB>(cl, cc, c, lo, hi, np, i, j);
         np = min(np, W);
         const int span = hi - lo;
         const int npv = c.vr - c.vl;
         const bool use_prof = in_lds && span <= pmax && npv <= W;
         T cv[W];
-        vweights<W>(c, lane, cv);
+        vweights<W, TAB>(P, c, lane, cv);
         T val = T(0);
         Vox<T> v;
         bool act = false;
-        voxel(P, a12, c, kc, v);
+        voxel<TAB>(P, a12, c, kc, v);
         act = kin && on_detector(v);
         val = kin ? x * v.rho : T(0);
         if (use_prof) {
@@ -794,11 +863,11 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           // lanes off the fast path add 0 into the margin, and the rare
           // voxels that need the direct path take it in a separate block.
           const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
-          profile_n<W>(np, v, fast ? prof + (v.ul - lo) : prof - W, fast ? val : T(0),
-                       plain);
+          profile_n<W, TAB>(P, np, v, fast ? prof + (v.ul - lo) : prof - W, fast ? val : T(0),
+                            plain);
           if (__ballot(act && !fast)) {
             T dummy = T(0);
-            if (act && !fast) slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
+            if (act && !fast) slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
           }
           __builtin_amdgcn_wave_barrier();
           // outer product P x cv; the lanes sit on distinct window columns and
@@ -820,7 +889,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           __builtin_amdgcn_wave_barrier();
         } else if (act) {
           T dummy = T(0);
-          slow_voxel<true, T>(P, c, v, val, out, nullptr, dummy);
+          slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
         }
       }
     }
@@ -948,7 +1017,7 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
 // VPL voxels per lane: the tile spans 64 VPL slices and lane L holds the
 // voxels k = kf + 64 q + L, q < VPL, so that the per-column set-up (column
 // broadcast, v weights, Q row) serves 64 VPL voxels.
-template <bool CONE, int W, int CX, int CY, bool TWO, int VPL, typename T>
+template <bool CONE, int W, int CX, int CY, bool TWO, int VPL, bool TAB, typename T>
 __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T *__restrict__ sino,
                                                 T *__restrict__ vol,
                                                 const int *__restrict__ seg_begin, int atomic_out,
@@ -988,8 +1057,8 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
         const int s2 = lane / NCOL;
         const bool active = n + s2 < seg.count;
         const int vw = P.order[seg.start + (active ? n + s2 : n)];
-        column_entry<CONE, CX, CY>(P, P.views[vw], tm, wave * NCOL + lane % NCOL, active, kf, kl,
-                                   cl);
+        column_entry<CONE, TAB, CX, CY>(P, P.views[vw], tm, wave * NCOL + lane % NCOL, active, kf,
+                                        kl, cl);
         if (cl.bad) atomicOr(P.err, 1);
       }
       const int view = P.order[seg.start + n];
@@ -1046,13 +1115,13 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
           if (!((live_mask >> cc) & 1ull)) continue;
           Col<T> c;
           int lo, hi, np, i, j;
-          bcast_col(cl, loff + cc, c, lo, hi, np, i, j);
+          bcast_col<TAB>(cl, loff + cc, c, lo, hi, np, i, j);
           np = min(np, W);
           const int span = hi - lo;
           const int npv = c.vr - c.vl;
           const bool use_q = in_lds && span <= pmax && npv <= W;
           T cv[W];
-          vweights<W>(c, lane, cv);
+          vweights<W, TAB>(P, c, lane, cv);
           T contrib[VPL];
 #pragma unroll
           for (int q = 0; q < VPL; ++q) contrib[q] = T(0);
@@ -1086,7 +1155,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               const int k = kf + kKT * q + lane;
               const bool kin = k < P.nz;
               Vox<T> v;
-              voxel(P, a12, c, kin ? k : P.nz - 1, v);
+              voxel<TAB>(P, a12, c, kin ? k : P.nz - 1, v);
               // All np pixels: zero weights past the footprint, zero Q past
               // the column span (right detector edge) and in the W entries
               // in front of the row (windows starting left of the detector).
@@ -1095,10 +1164,11 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               // in a separate block.
               const bool act = kin && on_detector(v);
               const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
-              const T gq = gather_n<W>(np, v, fast ? qrow + (v.ul - lo) : qrow - W);
+              const T gq = gather_n<W, TAB>(P, np, v, fast ? qrow + (v.ul - lo) : qrow - W);
               contrib[q] = fast ? gq * v.rho : T(0);
               if (__ballot(act && !fast)) {
-                if (act && !fast) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+                if (act && !fast)
+                  slow_voxel<false, TAB, T>(P, c, v, T(0), nullptr, in, contrib[q]);
               }
             }
             __builtin_amdgcn_wave_barrier();
@@ -1108,8 +1178,9 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               const int k = kf + kKT * q + lane;
               if (k < P.nz) {
                 Vox<T> v;
-                voxel(P, a12, c, k, v);
-                if (on_detector(v)) slow_voxel<false, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+                voxel<TAB>(P, a12, c, k, v);
+                if (on_detector(v))
+                  slow_voxel<false, TAB, T>(P, c, v, T(0), nullptr, in, contrib[q]);
               }
             }
           }
@@ -1195,19 +1266,26 @@ size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
 template <bool CONE, int W, int CX, int CY, typename T>
 void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *volk, T *out,
                 int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain) {
-  hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, T>), grid, dim3(64), lds_bytes, st, P, volk, out,
-                     vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
+  if (P.table)
+    hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, true, T>), grid, dim3(64), lds_bytes, st, P,
+                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
+  else
+    hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, false, T>), grid, dim3(64), lds_bytes, st, P,
+                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
 }
 
 template <bool CONE, int W, int CX, int CY, int VPL, typename T>
 void launch_bwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *in, T *out,
                 const int *seg_begin, int atomic_out, int lds_cap, int pmax, bool two) {
-  if (two)
-    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, true, VPL, T>), grid, dim3(axial::kThreads),
-                       lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax);
-  else
-    hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, false, VPL, T>), grid, dim3(axial::kThreads),
-                       lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax);
+#define TRTT_BWD(TW, TB)                                                                        \
+  hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, TW, VPL, TB, T>), grid, dim3(axial::kThreads), \
+                     lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax)
+  if (P.table) {
+    if (two) TRTT_BWD(true, true); else TRTT_BWD(false, true);
+  } else {
+    if (two) TRTT_BWD(true, false); else TRTT_BWD(false, false);
+  }
+#undef TRTT_BWD
 }
 
 template <bool CONE, int W, typename T>

@@ -42,6 +42,7 @@ hipError_t sparse_spmv(bool f64, const int *ptr, const int *idx, const void *val
 hipError_t launch_dd(bool f64, const DDConst *cs, const DDConst &c0, int nviews, int job,
                      int stuck, const void *in, void *out, hipStream_t st);
 bool host_column_2d(bool fan, const DevPlan &P, const ViewConst &g, int i, int j, double *out);
+double host_cdf3(double x);
 void host_lf_column(const DevPlan &P, const ViewConst &g, int i1, int i2, double *out);
 hipError_t launch_tinterp(bool f64, int job, const void *in, void *out, long npix, int nframes,
                           int taps, const int *idx, const void *w, const int *ptr,
@@ -185,11 +186,25 @@ void view_consts_lf(ViewConst &g, const lf2d_parameters_t *p, double beta, doubl
 // rotate about the object z axis (o3 = den_z = num_v_z = 0), with the whole
 // volume in front of the source.  Returns the footprint width bound W
 // (9 or 13 pixels) and the column profile length, or W = 0.
+// A footprint table that holds the cubic B-spline CDF at its nodes,
+// cum[n] = F(coord0 + n step) - F(coord0) to within 1e-6 (the trapezoid
+// tables of Ytrtt_tomobj.i:140-196 are within 4e-8 for 10^4 entries), lets
+// the axial kernels evaluate the table entry of an index in closed form.
+bool table_is_cubic_cdf(const double *cum, long m, double step, double coord0) {
+  if (!cum || m < 2 || !(step > 0.0)) return false;
+  const double f0 = trtt::host_cdf3(coord0);
+  for (long n = 0; n < m; ++n)
+    if (!(std::fabs(cum[n] - (trtt::host_cdf3(coord0 + double(n) * step) - f0)) <= 1e-6))
+      return false;
+  return true;
+}
+
 void axial_analysis(trtt_hip_plan *p) {
   p->axial_w = 0;
   p->axial_pmax = 0;
   p->axial_pmax2 = 0;
-  if (p->table || p->precision != TRTT_HIP_F32) return;
+  // tabulated footprints: only a cubic CDF table (closed form at the nodes)
+  if ((p->table && !(p->table_cdf3 && p->s_deg == 3)) || p->precision != TRTT_HIP_F32) return;
   if (p->kind != trtt::CB3D && p->kind != trtt::PB3D) return;
   // the kernels broadcast detector and column indices packed in 16 bits
   if (p->nu > 65535 || p->nv > 65535 || p->nx > 65535 || p->ny > 65535) return;
@@ -587,6 +602,7 @@ trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, lon
     release(p);
     return nullptr;
   }
+  p->table_cdf3 = p->table && table_is_cubic_cdf(cum, m, step, coord0);
   axial_analysis(p);
   if (p->axial_w > 0 && !alloc_axial_scratch(p)) {
     release(p);

@@ -119,6 +119,7 @@ struct trtt_hip_plan {
   int precision = TRTT_HIP_F32;
   int device = 0;
   bool table = false;
+  bool table_cdf3 = false;  // the table is the cubic B-spline CDF (axial path allowed)
   bool quirk_last_bin = false;
   long nx = 0, ny = 0, nz = 0, nt = 1, nu = 1, nv = 1, nviews = 0;
   double s_scl = 1.0, u_scl = 1.0, v_scl = 1.0;

@@ -871,6 +871,9 @@ bool host_column_2d(bool fan, const DevPlan &P, const ViewConst &g, int i, int j
   return true;
 }
 
+// The cubic B-spline CDF on the host (the function cdf3 of the kernels).
+double host_cdf3(double x) { return cdf3<double>(x); }
+
 void host_lf_column(const DevPlan &P, const ViewConst &g, int i1, int i2, double *out) {
   LFFoot f;
   lf_footprint(P, g, i1, i2, f);

@@ -199,7 +199,7 @@ class Projector:
     """
 
     def __init__(self, geom: Geometry, model: str = "cubic", precision: str = "f32", nt: int = 1,
-                 t0=None, t1=None, w0=None, w1=None, size_footprint: int = 10000):
+                 t0=None, t1=None, w0=None, w1=None, size_footprint: int = 10000, table=None):
         import torch  # device memory and streams only
 
         self.torch = torch
@@ -213,7 +213,12 @@ class Projector:
         xos, xsd = _f64(geom.xos), _f64(geom.xsd)
         self._keep = [xos, xsd]
         if model == "table":
-            _, cum, coord, step = footprint_table(size_footprint, geom.s_deg)
+            # the caller's (cum, coord, step), or the table of Ytrtt_tomobj.i:140-196
+            if table is not None:
+                cum, coord, step = table
+                cum, coord, step = _f64(cum), _f64(coord), float(step)
+            else:
+                _, cum, coord, step = footprint_table(size_footprint, geom.s_deg)
             self._keep.append(cum)
             cum_p, m, c0 = _ptr(cum), cum.size, float(coord[0])
         else:
