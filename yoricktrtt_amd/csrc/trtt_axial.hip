@@ -106,10 +106,13 @@ struct Col {
 // table's own trapezoid error; the host accepts a table for this path only
 // when it is within 1e-6 of that).  The index is taken in FP64 from the
 // reference's coordinate expressions, the CDF at the node in T.
+// The reference clamps the index to [0, m - 1]; the host accepts a table
+// for this path only when it covers the footprint's support [-2, 2], so
+// past either end the CDF at the node is the constant the clamped entry
+// holds (-1/2 or +1/2 here) and no clamp is needed.
 template <typename T>
 __device__ __forceinline__ T table_cdf(const DevPlan &P, double x) {
-  const double c = fmin(fmax(x, 0.0), double(P.m - 1));
-  return cdf3c<T>(T(fma(floor(c), P.step, P.coord0)));
+  return cdf3c<T>(fmaf(float(floor(x)), float(P.step), float(P.coord0)));
 }
 
 template <bool CONE, bool TAB, typename T>
@@ -216,12 +219,13 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
     // resolution)
     const double ud = (v.uc - a12) * P.u_scl;
     const double dsq = fma(ud * ud, c.inv_ad, 1.0);
-    double dud = double(fsqrt(float(dsq)));
-    dud = 0.5 * (dud + dsq / dud);
-    const double hud = c.hu0d * dud;
+    // 1/sqrt(dsq) from single precision, one Newton step (relative error
+    // about 2^-46), then du = dsq / sqrt(dsq)
+    double rs = double(__builtin_amdgcn_rsqf(float(dsq)));
+    rs = rs * fma(-0.5 * dsq * rs, rs, 1.5);
+    const double hud = c.hu0d * (dsq * rs);
     const double den = hud * P.step;
     double r = double(frcp(float(den)));
-    r = fma(r, fma(-den, r, 1.0), r);
     r = fma(r, fma(-den, r, 1.0), r);
     v.sdu = r;
     v.xu0 = ((double(v.ul) - v.uc) - hud * P.coord0) * r;

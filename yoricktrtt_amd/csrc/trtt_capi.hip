@@ -192,6 +192,8 @@ void view_consts_lf(ViewConst &g, const lf2d_parameters_t *p, double beta, doubl
 // the axial kernels evaluate the table entry of an index in closed form.
 bool table_is_cubic_cdf(const double *cum, long m, double step, double coord0) {
   if (!cum || m < 2 || !(step > 0.0)) return false;
+  // the table covers the support [-2, 2] (the axial kernels skip the clamp)
+  if (!(coord0 <= -2.0) || !(coord0 + double(m - 1) * step >= 2.0)) return false;
   const double f0 = trtt::host_cdf3(coord0);
   for (long n = 0; n < m; ++n)
     if (!(std::fabs(cum[n] - (trtt::host_cdf3(coord0 + double(n) * step) - f0)) <= 1e-6))
