@@ -12,6 +12,9 @@ and records inputs and outputs of every projector on small seeded cases:
                     pixel on the views of the 2-D geometries
 * ``dd3d.npz``      trtt_DD_PB3D, job 0 and 1, parallel geometries with pixels
                     smaller and larger than the voxels
+* ``deg2.npz``      trtt_{PB,CB}3D and trtt_{PB,FB}2D with s_deg = 2 and a
+                    degree-2 table (10^3 and 10^4 entries) on the geometries
+                    and inputs of proj3d / proj2d
 
 The per-view XOS/XSD pairs are composed with the reference transform code in
 the same sequence as the Yorick system builders
@@ -181,6 +184,81 @@ def ref_table(lib, m):
     cum = np.zeros(m)
     lib._trtt_cumsum(_p(cum), _p(fp), m, step)
     return fp, cum, coord, step
+
+
+def bspline2(x):
+    t = np.abs(x)
+    out = np.where(t < 0.5, 0.75 - t * t, 0.0)
+    return np.where((t >= 0.5) & (t < 1.5), 0.5 * (1.5 - t) ** 2, out)
+
+
+def ref_table2(lib, m):
+    """Degree-2 footprint table: samples of the quadratic B-spline on [-3, 3]
+    (n = 2 s_deg + 3 nodes, Ytrtt_tomobj.i:140-181) summed by the reference
+    _trtt_cumsum."""
+    step = 6.0 / (m - 1)
+    coord = np.arange(m) * step - 3.0
+    fp = bspline2(coord)
+    cum = np.zeros(m)
+    lib._trtt_cumsum(_p(cum), _p(fp), m, step)
+    return fp, cum, coord, step
+
+
+def deg2_cases(lib, proj3d, proj2d):
+    """s_deg = 2 with a degree-2 table (the 4-D benchmark script's setting,
+    Tomography_4D/Ytrtt4D_script_computation_time_benchmarking.i:19-42) on
+    the geometries and inputs of proj3d / proj2d: trtt_{PB,CB}3D and
+    trtt_{PB,FB}2D, job 0 and 1, tables of 10^3 and 10^4 entries."""
+    out = {}
+    for m in (1000, 10000):
+        fp, cum, coord, step = ref_table2(lib, m)
+        out[f"m{m}/cum"] = cum
+        out[f"m{m}/coord"] = coord
+        out[f"m{m}/step"] = np.array(step)
+        vol = np.ascontiguousarray(proj3d["vol"])
+        nz, ny, nx = vol.shape
+        s = float(proj3d["s"])
+        for name in ("cone_a", "cone_edge", "par_a"):
+            nu, nv = (int(v) for v in proj3d[f"{name}/dims"])
+            us, vs = (float(v) for v in proj3d[f"{name}/scl"])
+            xos, xsd = proj3d[f"{name}/xos"], proj3d[f"{name}/xsd"]
+            sino_in = proj3d[f"{name}/sino_in"]
+            nview = xos.shape[0]
+            fn = getattr(lib, "trtt_CB3D" if name.startswith("cone") else "trtt_PB3D")
+            fwd = np.zeros((nview, nv, nu))
+            bwd = np.zeros((nview, nz, ny, nx))
+            for v in range(nview):
+                for job in (0, 1):
+                    o = fwd[v] if job == 0 else bwd[v]
+                    i = np.ascontiguousarray(vol if job == 0 else sino_in[v])
+                    r = fn(_p(o), _p(i), nx, ny, nz, s, 2, _p(fp), _p(cum), m, step, _p(coord),
+                           nu, nv, us, vs, _p(np.ascontiguousarray(xos[v])),
+                           _p(np.ascontiguousarray(xsd[v])), job)
+                    assert r == 0
+            out[f"m{m}/3d/{name}/fwd"] = fwd
+            out[f"m{m}/3d/{name}/bwd"] = bwd
+        img = np.ascontiguousarray(proj2d["img"])
+        ny2, nx2 = img.shape
+        for name in ("fan_a", "fan_edge", "par_a"):
+            nv = int(proj2d[f"{name}/nv"])
+            vs = float(proj2d[f"{name}/vs"])
+            xos, xsd = proj2d[f"{name}/xos"], proj2d[f"{name}/xsd"]
+            sino_in = proj2d[f"{name}/sino_in"]
+            nview = xos.shape[0]
+            fn = getattr(lib, "trtt_FB2D" if name.startswith("fan") else "trtt_PB2D")
+            fwd = np.zeros((nview, nv))
+            bwd = np.zeros((nview, ny2, nx2))
+            for v in range(nview):
+                for job in (0, 1):
+                    o = fwd[v] if job == 0 else bwd[v]
+                    i = np.ascontiguousarray(img if job == 0 else sino_in[v])
+                    r = fn(_p(o), _p(i), nx2, ny2, float(proj2d["s"]), 2, _p(fp), _p(cum), m,
+                           step, _p(coord), nv, vs, _p(np.ascontiguousarray(xos[v])),
+                           _p(np.ascontiguousarray(xsd[v])), job)
+                    assert r == 0
+            out[f"m{m}/2d/{name}/fwd"] = fwd
+            out[f"m{m}/2d/{name}/bwd"] = bwd
+    return out
 
 
 def xform_cases(lib):
@@ -399,7 +477,7 @@ def main(names=None):
     if not os.path.exists(LIB):
         sys.exit(f"{LIB} missing: run `make -C oracle` where the reference sources exist")
     lib = load()
-    names = names or ["xform", "footprint", "proj3d", "proj2d", "sparse2d", "dd3d"]
+    names = names or ["xform", "footprint", "proj3d", "proj2d", "sparse2d", "dd3d", "deg2"]
     if "xform" in names:
         np.savez_compressed(os.path.join(HERE, "xform.npz"), **xform_cases(lib))
     if "footprint" in names:
@@ -417,6 +495,10 @@ def main(names=None):
         np.savez_compressed(os.path.join(HERE, "sparse2d.npz"), **sparse2d_cases(lib, proj2d))
     if "dd3d" in names:
         np.savez_compressed(os.path.join(HERE, "dd3d.npz"), **dd3d_cases(lib))
+    if "deg2" in names:
+        proj3d = np.load(os.path.join(HERE, "proj3d.npz"))
+        proj2d = np.load(os.path.join(HERE, "proj2d.npz"))
+        np.savez_compressed(os.path.join(HERE, "deg2.npz"), **deg2_cases(lib, proj3d, proj2d))
     for f in names:
         print(f, os.path.getsize(os.path.join(HERE, f + ".npz")), "bytes")
 

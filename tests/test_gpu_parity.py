@@ -698,3 +698,119 @@ def test_axial_table_cone_vs_oracle(lib, oracle, m):
                     tab)
     assert rel_err(bwd, rb) < 1e-5
     P.close()
+
+
+@pytest.mark.parametrize("m", [1000, 10000])
+def test_deg2_tables_vs_reference(lib, golden3d, golden2d, precision, m):
+    """s_deg = 2 with a degree-2 table (the reference's 4-D benchmark
+    setting): the drop-in symbols and the batched plans against the
+    reference C outputs of tests/golden/deg2.npz."""
+    import os
+
+    from conftest import GOLDEN
+    from yoricktrtt_amd.geometry import Geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("deg2", m, precision)
+    d = dict(np.load(os.path.join(GOLDEN, "deg2.npz")))
+    cum, coord, step = d[f"m{m}/cum"], d[f"m{m}/coord"], float(d[f"m{m}/step"])
+    fp = np.zeros_like(cum)
+    tol = TOL[precision]
+    dt = torch.float64 if precision == "f64" else torch.float32
+    vol = np.ascontiguousarray(golden3d["vol"])
+    nz, ny, nx = vol.shape
+    for name in ("cone_a", "cone_edge", "par_a"):
+        nu, nv = (int(v) for v in golden3d[f"{name}/dims"])
+        us, vs = (float(v) for v in golden3d[f"{name}/scl"])
+        xos, xsd = golden3d[f"{name}/xos"], golden3d[f"{name}/xsd"]
+        sino_in = golden3d[f"{name}/sino_in"]
+        cone = name.startswith("cone")
+        fn = lib.trtt_CB3D if cone else lib.trtt_PB3D
+        rf, rb = d[f"m{m}/3d/{name}/fwd"], d[f"m{m}/3d/{name}/bwd"]
+        for v in range(xos.shape[0]):
+            xo, xs = np.ascontiguousarray(xos[v]), np.ascontiguousarray(xsd[v])
+            of = np.zeros((nv, nu))
+            ob = np.zeros((nz, ny, nx))
+            assert fn(_p(of), _p(vol), nx, ny, nz, 1.0, 2, _p(fp), _p(cum), m, step, _p(coord), nu,
+                      nv, us, vs, _p(xo), _p(xs), 0) == 0
+            assert fn(_p(ob), _p(np.ascontiguousarray(sino_in[v])), nx, ny, nz, 1.0, 2, _p(fp),
+                      _p(cum), m, step, _p(coord), nu, nv, us, vs, _p(xo), _p(xs), 1) == 0
+            assert rel_err(of, rf[v]) < tol and rel_err(ob, rb[v]) < tol, (name, v)
+        geo = Geometry("cone" if cone else "parallel3d", nx, ny, nz, nu, nv, 1.0, us, vs,
+                       np.ascontiguousarray(xos), np.ascontiguousarray(xsd), s_deg=2)
+        P = Projector(geo, model="table", precision=precision, table=(cum, coord, step))
+        assert P.path == 0  # a degree-2 footprint runs the general kernels
+        fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
+        bwd = P.backward(_dev(sino_in, dt)).double().cpu().numpy()
+        assert rel_err(fwd, rf) < tol and rel_err(bwd, rb.sum(axis=0)) < tol, name
+        P.close()
+    img = np.ascontiguousarray(golden2d["img"])
+    ny2, nx2 = img.shape
+    for name in ("fan_a", "fan_edge", "par_a"):
+        nv, vs = int(golden2d[f"{name}/nv"]), float(golden2d[f"{name}/vs"])
+        xos, xsd = golden2d[f"{name}/xos"], golden2d[f"{name}/xsd"]
+        sino_in = golden2d[f"{name}/sino_in"]
+        fan = name.startswith("fan")
+        fn = lib.trtt_FB2D if fan else lib.trtt_PB2D
+        rf, rb = d[f"m{m}/2d/{name}/fwd"], d[f"m{m}/2d/{name}/bwd"]
+        for v in range(xos.shape[0]):
+            xo, xs = np.ascontiguousarray(xos[v]), np.ascontiguousarray(xsd[v])
+            of = np.zeros(nv)
+            ob = np.zeros((ny2, nx2))
+            assert fn(_p(of), _p(img), nx2, ny2, 1.0, 2, _p(fp), _p(cum), m, step, _p(coord), nv,
+                      vs, _p(xo), _p(xs), 0) == 0
+            assert fn(_p(ob), _p(np.ascontiguousarray(sino_in[v])), nx2, ny2, 1.0, 2, _p(fp),
+                      _p(cum), m, step, _p(coord), nv, vs, _p(xo), _p(xs), 1) == 0
+            assert rel_err(of, rf[v]) < tol and rel_err(ob, rb[v]) < tol, (name, v)
+
+
+@pytest.mark.parametrize("case", ["fan_f32", "lf2d_f32", "cone_f64", "tilted_f32"])
+def test_atomic_forwards_run_to_run(lib, case):
+    """The general forwards sum each tile's window in LDS with atomics and add
+    it to the sinogram with global atomics, so the summation order, and the
+    last bits of an FP32 result, vary between runs (a few FP32 roundings per
+    pixel): three runs agree within 1e-6 relative (FP32) or 1e-14 (FP64)."""
+    from yoricktrtt_amd import _lib
+    from yoricktrtt_amd.geometry import (Geometry, XForm3D, cone_geometry, detector_xi,
+                                         fan_geometry_2d, object_xi, projector_coefs,
+                                         view_transforms_3d)
+    from yoricktrtt_amd.projectors import LFProjector, Projector
+
+    _say("run to run", case)
+    rng = np.random.default_rng(77)
+    if case == "fan_f32":
+        geo = fan_geometry_2d(300, 280, 1.0, 420, 1.0, np.arange(180) * (2 * math.pi / 180), 500.0,
+                              800.0)
+        P = Projector(geo, model="cubic", precision="f32")
+        x = rng.uniform(size=(280, 300))
+    elif case == "lf2d_f32":
+        V = 180
+        P = LFProjector(_lib.LF2DParameters(0.75, 0.75, 0.5, 400.0, 200.0, 0.0, 0.0, 300, 280, 420),
+                        np.arange(V) * (2 * math.pi / V), np.zeros(V), precision="f32")
+        x = rng.uniform(size=(280, 300))
+    elif case == "cone_f64":
+        geo = cone_geometry(60, 56, 50, 1.0, 80, 90, 1.0, 1.0, np.arange(24) * 0.26, 150.0, 220.0)
+        P = Projector(geo, model="cubic", precision="f64")
+        x = rng.uniform(size=(50, 56, 60))
+    else:
+        nx, ny, nz, nu, nv, V = 48, 44, 40, 64, 70, 12
+        xs, ds = [], []
+        for th in np.linspace(0.2, 5.0, V):
+            xos, xot, xsd = view_transforms_3d(th, 120.0, 190.0)
+            xos = XForm3D.combine(xos, XForm3D().rotate(math.cos(0.1), math.sin(0.1), 0.0, 0.0))
+            a, b = projector_coefs(object_xi(nx, ny, nz, 1.0), detector_xi(nu, nv, 1.0, 1.0), xos,
+                                   xot, xsd)
+            xs.append(a)
+            ds.append(b)
+        geo = Geometry("cone", nx, ny, nz, nu, nv, 1.0, 1.0, 1.0, np.array(xs), np.array(ds))
+        P = Projector(geo, model="cubic", precision="f32")
+        x = rng.uniform(size=(nz, ny, nx))
+    assert P.path == 0  # the general kernels
+    dt = P.dtype
+    xd = _dev(x, dt)
+    runs = [P.forward(xd).double().cpu().numpy() for _ in range(3)]
+    tol = 1e-14 if dt == torch.float64 else 1e-6
+    worst = max(rel_err(r, runs[0]) for r in runs[1:])
+    _say(f"  run-to-run {worst:.2e}")
+    assert worst < tol
+    P.close()

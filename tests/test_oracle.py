@@ -8,12 +8,13 @@
   adjoint test of tomography-LongFessler-model/tomo_model.i:174-209.
 """
 import math
+import os
 
 import numpy as np
 import pytest
 from scipy import integrate
 
-from conftest import CASES2D, CASES3D, rel_err
+from conftest import CASES2D, CASES3D, GOLDEN, rel_err
 
 
 def _table(g):
@@ -251,3 +252,45 @@ def test_lf_clear_flag(oracle):
     a = oracle.lf2d(p, 0.2, 0.0, x, dst.copy(), 0, 0)
     b = oracle.lf2d(p, 0.2, 0.0, x, dst.copy(), 1, 0)
     assert np.allclose(a - 5.0, b, rtol=0, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# s_deg = 2 with a degree-2 table (tests/golden/deg2.npz, from the reference C)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("m", [1000, 10000])
+def test_oracle_deg2_tables_vs_reference(oracle, golden3d, golden2d, m):
+    from yoricktrtt_amd.geometry import footprint_table
+
+    d = dict(np.load(os.path.join(GOLDEN, "deg2.npz")))
+    cum, coord, step = d[f"m{m}/cum"], d[f"m{m}/coord"], float(d[f"m{m}/step"])
+    # the host table restatement equals the reference's degree-2 table
+    _, c2, co2, st2 = footprint_table(m, 2)
+    assert np.abs(c2 - cum).max() < 1e-15 and abs(st2 - step) < 1e-16
+    tab = (cum, coord, step)
+    vol = golden3d["vol"]
+    nz, ny, nx = vol.shape
+    for name in ("cone_a", "cone_edge", "par_a"):
+        nu, nv = (int(v) for v in golden3d[f"{name}/dims"])
+        us, vs = (float(v) for v in golden3d[f"{name}/scl"])
+        xos, xsd = golden3d[f"{name}/xos"], golden3d[f"{name}/xsd"]
+        cone = name.startswith("cone")
+        for v in range(xos.shape[0]):
+            rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, vs, xos[v],
+                             xsd[v], cone, 0, tab, deg=2)
+            assert rel_err(rf, d[f"m{m}/3d/{name}/fwd"][v]) < 1e-14, (name, v)
+            rb = oracle.sb3d(np.zeros((nz, ny, nx)), golden3d[f"{name}/sino_in"][v], nx, ny, nz,
+                             1.0, nu, nv, us, vs, xos[v], xsd[v], cone, 1, tab, deg=2)
+            assert rel_err(rb, d[f"m{m}/3d/{name}/bwd"][v]) < 1e-14, (name, v)
+    img = golden2d["img"]
+    ny2, nx2 = img.shape
+    for name in ("fan_a", "fan_edge", "par_a"):
+        nv, vs = int(golden2d[f"{name}/nv"]), float(golden2d[f"{name}/vs"])
+        xos, xsd = golden2d[f"{name}/xos"], golden2d[f"{name}/xsd"]
+        fan = name.startswith("fan")
+        for v in range(xos.shape[0]):
+            rf = oracle.sb2d(np.zeros(nv), img, nx2, ny2, 1.0, nv, vs, xos[v], xsd[v], fan, 0,
+                             tab, deg=2)
+            assert rel_err(rf, d[f"m{m}/2d/{name}/fwd"][v]) < 1e-14, (name, v)
+            rb = oracle.sb2d(np.zeros((ny2, nx2)), golden2d[f"{name}/sino_in"][v], nx2, ny2, 1.0,
+                             nv, vs, xos[v], xsd[v], fan, 1, tab, deg=2)
+            assert rel_err(rb, d[f"m{m}/2d/{name}/bwd"][v]) < 1e-14, (name, v)
