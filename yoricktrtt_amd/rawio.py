@@ -4,7 +4,8 @@ Yorick's ``raw_read(file, type, d1, d2, ..., offset=n)`` reads
 d1*d2*... values of ``type`` in native byte order after n header bytes,
 d1 varying fastest.  The scripts use it for
 * 3-D projections: ``raw_read(data_file, float, nu, nv, 720, offset=512)``
-  then every ``each``-th view (Tomography_3D/Ytrtt3D_script_reconstructions.i:50);
+  then every ``each``-th view (Tomography_3D/Ytrtt3D_script_reconstructions.i:50),
+  each view reversed along u and transposed (:52-54) on request;
 * 3-D reference volumes: ``raw_read(object_file, float, nx, ny, nz,
   offset=512)`` (same script, :51);
 * 2-D real projections: ``raw_read(file, short, 512, 512, offset=100)
@@ -42,13 +43,25 @@ def raw_write(path, array, offset: int = 0, ytype: str = None) -> None:
 
 
 def read_projections_3d(path, nu: int, nv: int, nstored: int = 720, each: int = 1,
-                        ndata: int = None, offset: int = 512) -> np.ndarray:
+                        ndata: int = None, offset: int = 512,
+                        reference_orientation: bool = False) -> np.ndarray:
     """[ndata][nv][nu] float64: every ``each``-th of ``nstored`` FP32 views,
-    the first ``ndata`` of them (Ytrtt3D_script_reconstructions.i:50)."""
+    the first ``ndata`` of them (Ytrtt3D_script_reconstructions.i:50).
+
+    With ``reference_orientation`` every view is also reversed along u and
+    transposed, as the script does before reconstructing
+    (data(..,k) = transpose(data(::-1,,k)), :52-54): in this package's
+    [nv][nu] order, out[k][j][i] = stored[k][i][nu - 1 - j].  The script's
+    in-place assignment needs a square detector, and so does this option."""
     data = raw_read(path, "float", (nu, nv, nstored), offset)[::each]
     if ndata is not None:
         data = data[:ndata]
-    return np.asarray(data, dtype=np.float64)
+    out = np.asarray(data, dtype=np.float64)
+    if reference_orientation:
+        if nu != nv:
+            raise ValueError(f"reference_orientation needs nu == nv (got {nu} x {nv})")
+        out = np.ascontiguousarray(out[:, :, ::-1].transpose(0, 2, 1))
+    return out
 
 
 def read_volume(path, nx: int, ny: int, nz: int, offset: int = 512) -> np.ndarray:
