@@ -1,5 +1,6 @@
-"""Learned warm start against the plain VMLM-B solve on held-out objects
-(seeds 12000 + j, never seen in training), with the committed networks.
+"""Learned warm start against the plain VMLM-B solve on 100 held-out objects
+(seeds 12000 + j, j < 100, never seen in training), with the committed
+networks.
 
 Measured here, per object: the plain 50-iteration objective f50, the
 iterations and wall time the plain solve needs to come within 1e-3 of f50,
@@ -39,7 +40,7 @@ def test_trained_network_gains_over_plain(rec, which):
     _say("warm start vs plain on held-out objects:", which)
     net = UNet3D.load(DEFAULT_WEIGHTS if which == "default" else IMAGE_WEIGHTS)
     assert net.in_ch == (2 if which == "default" else 1)
-    rows = evaluate(rec, net, [EVAL_SEED0 + j for j in range(4)], log=lambda m: _say(" ", m))
+    rows = evaluate(rec, net, [EVAL_SEED0 + j for j in range(100)], log=lambda m: _say(" ", m))
     s = summarize(rows)
     _say(" ", s)
     for r in rows:

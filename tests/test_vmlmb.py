@@ -80,3 +80,68 @@ def test_checkpoint_resume_continues_the_same_iterates(tmp_path):
     rest = vmlmb(fg, x, lower=0.0, mem=5, maxiter=18 - niter, grtol=1e-3, pairs=pairs, g0norm=g0)
     assert torch.equal(rest.x, full.x)
     assert rest.history[1:] == full.history[8:]
+
+
+# ---------------------------------------------------------------------------
+# More-Thuente line search: the test functions of More & Thuente (1994),
+# section 5; from start steps 1e-3 .. 1e3 the search must return a step that
+# satisfies the strong Wolfe conditions it guarantees.
+# ---------------------------------------------------------------------------
+def _mt_funcs():
+    import math as m
+
+    def f1(a, b=2.0):
+        return -a / (a * a + b), (a * a - b) / (a * a + b) ** 2
+
+    def f2(a, b=0.004):
+        return (a + b) ** 5 - 2.0 * (a + b) ** 4, 5.0 * (a + b) ** 4 - 8.0 * (a + b) ** 3
+
+    def f3(a, b=0.01, l=39):
+        if a <= 1.0 - b:
+            p0, d0 = 1.0 - a, -1.0
+        elif a >= 1.0 + b:
+            p0, d0 = a - 1.0, 1.0
+        else:
+            p0, d0 = (a - 1.0) ** 2 / (2.0 * b) + 0.5 * b, (a - 1.0) / b
+        w = l * m.pi / 2.0
+        return (p0 + 2.0 * (1.0 - b) / (l * m.pi) * m.sin(w * a),
+                d0 + (1.0 - b) * m.cos(w * a))
+
+    return [(f1, 1e-3, 0.1), (f2, 0.1, 0.1), (f3, 0.1, 0.1)]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+@pytest.mark.parametrize("a0", [1e-3, 1e-1, 1e1, 1e3])
+def test_more_thuente_strong_wolfe(case, a0):
+    from yoricktrtt_amd.vmlmb import more_thuente
+
+    phi, ftol, gtol = _mt_funcs()[case]
+    f0, g0 = phi(0.0)
+    # the paper's experiments: a small interval tolerance, no flat acceptance
+    stp, f, g, nfev, status = more_thuente(phi, a0, f0, g0, ftol=ftol, gtol=gtol, xtol=1e-10,
+                                           feps=0.0)
+    assert status == "converged", (status, stp, nfev)
+    assert f <= f0 + ftol * stp * g0
+    assert abs(g) <= gtol * abs(g0)
+    assert (f, g) == phi(stp)
+    assert nfev <= 30
+
+
+def test_vmlmb_line_searches_agree_on_rosenbrock():
+    """Both line searches reach the minimiser of the 2-D Rosenbrock function
+    (1, 1); More-Thuente with fewer evaluations."""
+    from yoricktrtt_amd.vmlmb import vmlmb
+
+    def fg(x, g):
+        a, b = x[0].item(), x[1].item()
+        g[0] = -2.0 * (1.0 - a) - 400.0 * a * (b - a * a)
+        g[1] = 200.0 * (b - a * a)
+        return (1.0 - a) ** 2 + 100.0 * (b - a * a) ** 2
+
+    runs = {}
+    for ls in ("more-thuente", "armijo"):
+        r = vmlmb(fg, torch.tensor([-1.2, 1.0], dtype=torch.float64), maxiter=2000, gatol=1e-10,
+                  linesearch=ls)
+        assert abs(r.x[0].item() - 1.0) < 1e-8 and abs(r.x[1].item() - 1.0) < 1e-8, ls
+        runs[ls] = r.neval
+    assert runs["more-thuente"] < runs["armijo"]
