@@ -21,18 +21,20 @@ and results.  Configuration 5 (the learned warm start) lives in
    the GPU from the tabulated footprint, products timed, its own solve).
 3. 3-D cone beam, 256^3 voxels of 1 mm, 360 views over 360 deg, 512 (v,
    transaxial) x 384 (u, axial) flat detector at 1 mm, Rsc = 1000 mm,
-   Rsd = 1500 mm, FP32; 30 random ellipsoids (seed 3); data = this
-   projector's forward projection with Poisson noise (1e5 photons,
-   0.02 mm^-1 per unit; the cone projector's output is the footprint
-   integrated over each detector pixel, about pitch^2 times the line
-   integral in mm); 30 iterations.
+   Rsd = 1500 mm, FP32; 30 random ellipsoids (seed 3); data = the
+   analytic cone-beam line integrals of the ellipsoids averaged over each
+   detector pixel (phantoms.cone_projection_ellipsoids, in the projector's
+   units: pitch^2 times the integral in mm) with Poisson noise (1e5
+   photons, 0.02 mm^-1 per unit); 30 iterations.
 4. 4-D cone beam: 10 phases of 128 x 128 x 64 voxels of 2 mm, 256 (v) x 192
    (u) detector at 2 mm, distances of configuration 3; view counts per phase
    uniform in 30..60 (seed 4), angles evenly spaced over 360 deg and dealt
    to the phases in a seeded random order; a body ellipsoid whose inner
-   ellipsoid moves by 10 sin(2 pi p / 10) mm along z; all phases in one
-   plan (one launch sequence per projection, the data term captured in one
-   HIP graph per evaluation), TV along t as well; 30 iterations.
+   ellipsoid moves by 10 sin(2 pi p / 10) mm along z; data = the analytic
+   cone-beam integrals of each view's phase (as for configuration 3) with
+   Poisson noise; all phases in one plan (one launch sequence per
+   projection, the data term captured in one HIP graph per evaluation), TV
+   along t as well; 30 iterations.
 6. 2-D+t fan beam (the dynamic scripts of Tomography_2D_dyn): 16 phases of
    256 x 256 pixels of 1 mm, a moving inner ellipse; 720 views over 360 deg
    acquired during 127 s of a 7 s motion; 512 bins of 1 mm, Rsc = 1000 mm,
@@ -140,16 +142,40 @@ def _sampler(shape, dtype, device):
     return sample
 
 
-def config1(iters=20, tv_weight=0.05, tv_eps=0.01):
+def config1_setup():
+    """Geometry of configuration 1: 2-D parallel, 256^2, 360 views / 180 deg,
+    367 unit bins."""
     from .geometry import parallel_geometry_2d
+
+    n, nv, nviews = 256, 367, 360
+    theta = np.arange(nviews) * (math.pi / nviews)
+    return parallel_geometry_2d(n, n, 1.0, nv, 1.0, theta)
+
+
+def config2_setup():
+    """Geometry of configuration 2 (2-D fan, 512^2 x 0.5 mm, 720 views /
+    360 deg, 736 flat bins x 0.75 mm, Rsc 1000, Rsd 1500) and the
+    parameters of its Long & Fessler projector (detector on the rotating
+    frame at Rsd - Rsc from the centre, bin pitch = bin size; source angle =
+    view angle - pi/2)."""
+    from . import _lib
+    from .geometry import fan_geometry_2d
+
+    n, s, nv, vs, nviews, Rsc, Rsd = 512, 0.5, 736, 0.75, 720, 1000.0, 1500.0
+    theta = np.arange(nviews) * (2 * math.pi / nviews)
+    geo = fan_geometry_2d(n, n, s, nv, vs, theta, Rsc, Rsd)
+    prm = _lib.LF2DParameters(vs, vs, s, Rsc, Rsd - Rsc, 0.0, 0.0, n, n, nv)
+    return geo, prm, theta - 0.5 * math.pi
+
+
+def config1(iters=20, tv_weight=0.05, tv_eps=0.01):
     from .phantoms import ellipses_2d, radon_ellipses_parallel
     from .projectors import Projector
     from .reconstruction import DataTerm, Objective, TotalVariation
 
     torch = _torch()
-    n, nv, nviews = 256, 367, 360
-    theta = np.arange(nviews) * (math.pi / nviews)
-    geo = parallel_geometry_2d(n, n, 1.0, nv, 1.0, theta)
+    geo = config1_setup()
+    n, nv, theta = geo.nx, geo.nv, geo.theta
     img, ells = ellipses_2d(n, 10, seed=1)
     v = (np.arange(nv) - (nv - 1) / 2) * 1.0
     y = radon_ellipses_parallel(ells, theta - 0.5 * math.pi, v, 1.0)
@@ -170,16 +196,14 @@ def config1(iters=20, tv_weight=0.05, tv_eps=0.01):
 
 
 def config2(iters=50, tv_weight=2.0, tv_eps=0.01, photons=1e5, mu_scale=0.02):
-    from . import _lib
-    from .geometry import fan_geometry_2d
     from .phantoms import ellipses_2d, radon_ellipses_fan, transmission_noise
     from .projectors import LFProjector, Projector
     from .reconstruction import DataTerm, Objective, TotalVariation
 
     torch = _torch()
     n, s, nv, vs, nviews, Rsc, Rsd = 512, 0.5, 736, 0.75, 720, 1000.0, 1500.0
-    theta = np.arange(nviews) * (2 * math.pi / nviews)
-    geo = fan_geometry_2d(n, n, s, nv, vs, theta, Rsc, Rsd)
+    geo, prm, beta = config2_setup()
+    theta = geo.theta
     img, ells = ellipses_2d(n, 20, seed=2)
     ells_mm = ells.copy()
     ells_mm[:, :4] *= s  # pixel units -> mm
@@ -207,11 +231,8 @@ def config2(iters=50, tv_weight=2.0, tv_eps=0.01, photons=1e5, mu_scale=0.02):
     out = dict(config=2, workload="2D fan 512^2 x 0.5 mm, 720 views / 360 deg, 736 flat bins "
                "0.75 mm, Rsc 1000, Rsd 1500, FP32, 1e5 photons",
                fwd_ms=1e3 * fwd, bwd_ms=1e3 * bwd, pairs_per_s=1.0 / (fwd + bwd), **sol)
-    # Long & Fessler SF projector on the same geometry (detector on the
-    # rotating frame at Rsd - Rsc from the centre, bin pitch = bin size); its
-    # source angle beta is the view angle minus pi/2
-    prm = _lib.LF2DParameters(vs, vs, s, Rsc, Rsd - Rsc, 0.0, 0.0, n, n, nv)
-    lf = LFProjector(prm, theta - 0.5 * math.pi, np.zeros(nviews), precision="f32")
+    # Long & Fessler SF projector on the same geometry (config2_setup)
+    lf = LFProjector(prm, beta, np.zeros(nviews), precision="f32")
     sino_lf = torch.zeros(lf.sino_shape, dtype=torch.float32, device=dev)
     lf_fwd, lf_bwd = _projection_times(lf, truth, sino_lf)
     xr = torch.rand((n, n), generator=torch.Generator(device=dev).manual_seed(7), device=dev)
@@ -279,7 +300,7 @@ def cone_setup(n=256, nviews=360, nu=384, nv=512, pitch=1.0, Rsc=1000.0, Rsd=150
 
 
 def config3(iters=30, tv_weight=5.0, tv_eps=0.05, photons=1e5, mu_scale=0.02):
-    from .phantoms import ellipsoids_3d, transmission_noise
+    from .phantoms import cone_projection_ellipsoids, ellipsoids_3d, transmission_noise
     from .projectors import Projector
     from .reconstruction import DataTerm, Objective, TotalVariation
 
@@ -287,15 +308,18 @@ def config3(iters=30, tv_weight=5.0, tv_eps=0.05, photons=1e5, mu_scale=0.02):
     n = 256
     geo = cone_setup(n)
     dev = torch.device("cuda")
-    truth = torch.from_numpy(ellipsoids_3d((n, n, n), 30, seed=3)).to(dev)
+    vol, ells = ellipsoids_3d((n, n, n), 30, seed=3, params=True)
+    truth = torch.from_numpy(vol).to(dev)
     P = _tuned(Projector(geo, model="cubic", precision="f32"))
     sino = torch.zeros(P.sino_shape, dtype=torch.float32, device=dev)
     fwd, bwd = _projection_times(P, truth, sino)
-    sino.zero_()
-    P.forward(truth, sino)
-    gen = torch.Generator(device=dev).manual_seed(3)
-    y, w = transmission_noise(sino, photons, mu_scale, generator=gen)
     del sino
+    # analytic data of the 30 ellipsoids (pixel-averaged cone-beam line
+    # integrals), not this projector's own output
+    data = cone_projection_ellipsoids(geo, ells, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(3)
+    y, w = transmission_noise(data, photons, mu_scale, generator=gen)
+    del data
     D = DataTerm(P, y, w)
     tv = TotalVariation((n, n, n), tv_weight, tv_eps, samples=True, dtype=torch.float32,
                         device=dev)
@@ -304,33 +328,41 @@ def config3(iters=30, tv_weight=5.0, tv_eps=0.05, photons=1e5, mu_scale=0.02):
                     _sampler((n, n, n), torch.float32, dev))
     P.close()
     return dict(config=3, workload="3D cone 256^3 x 1 mm, 360 views / 360 deg, 512x384 flat 1 mm, "
-                "Rsc 1000, Rsd 1500, FP32, 1e5 photons",
+                "Rsc 1000, Rsd 1500, FP32, analytic ellipsoid data, 1e5 photons",
                 fwd_ms=1e3 * fwd, bwd_ms=1e3 * bwd, pairs_per_s=1.0 / (fwd + bwd), **sol)
 
 
-def respiratory_series(nt=10, shape=(64, 128, 128), voxel=2.0, amplitude=10.0, seed=4):
+def respiratory_series(nt=10, shape=(64, 128, 128), voxel=2.0, amplitude=10.0, seed=4,
+                       params=False):
     """Body ellipsoid with fixed structures and an inner ellipsoid moving by
-    amplitude sin(2 pi p / nt) mm along z, for phases p = 0..nt-1."""
+    amplitude sin(2 pi p / nt) mm along z, for phases p = 0..nt-1.  With
+    ``params`` also the ellipsoids of every phase (centre, axes, rotation,
+    value in grid units), for the analytic data."""
     from .phantoms import add_ellipsoid
 
     rng = np.random.default_rng(seed)
     nz, ny, nx = shape
     eye = np.eye(3)
     base = np.zeros(shape, dtype=np.float64)
-    add_ellipsoid(base, (0.0, 0.0, 0.0), (0.42 * nx, 0.32 * ny, 0.45 * nz), eye, 0.4)
+    rows = [(np.zeros(3), np.array([0.42 * nx, 0.32 * ny, 0.45 * nz]), eye, 0.4)]
     for _ in range(6):
         c = rng.uniform(-0.25, 0.25, size=3) * np.array([nx, ny, nz])
         a = rng.uniform(0.04, 0.1, size=3) * np.array([nx, ny, nz])
-        add_ellipsoid(base, c, a, eye, rng.uniform(0.1, 0.5))
+        rows.append((c, a, eye, rng.uniform(0.1, 0.5)))
+    for c, a, r, v in rows:
+        add_ellipsoid(base, c, a, r, v)
     lesion_c = np.array([0.12 * nx, -0.05 * ny, 0.0])
     lesion_a = np.array([0.08 * nx, 0.08 * ny, 0.12 * nz])
     series = np.empty((nt,) + shape, dtype=np.float32)
+    phases = []
     for p in range(nt):
         vol = base.copy()
         dz = amplitude * math.sin(2 * math.pi * p / nt) / voxel
-        add_ellipsoid(vol, lesion_c + np.array([0.0, 0.0, dz]), lesion_a, eye, 0.5)
+        lc = lesion_c + np.array([0.0, 0.0, dz])
+        add_ellipsoid(vol, lc, lesion_a, eye, 0.5)
         series[p] = vol
-    return series
+        phases.append(rows + [(lc, lesion_a, eye, 0.5)])
+    return (series, phases) if params else series
 
 
 def config4_setup(nt=10, seed=4):
@@ -347,7 +379,7 @@ def config4_setup(nt=10, seed=4):
 
 
 def config4(iters=30, tv_weight=5.0, tv_eps=0.05, wt=1.0, photons=1e5, mu_scale=0.005):
-    from .phantoms import transmission_noise
+    from .phantoms import cone_projection_ellipsoids, transmission_noise
     from .projectors import Projector
     from .reconstruction import DataTerm, Objective, TotalVariation
 
@@ -355,18 +387,21 @@ def config4(iters=30, tv_weight=5.0, tv_eps=0.05, wt=1.0, photons=1e5, mu_scale=
     nt = 10
     geo, phase, counts = config4_setup(nt)
     dev = torch.device("cuda")
-    truth = torch.from_numpy(respiratory_series(nt)).to(dev)
+    series, phase_ells = respiratory_series(nt, params=True)
+    truth = torch.from_numpy(series).to(dev)
     P = _tuned(Projector(geo, model="cubic", precision="f32", nt=nt, t0=phase, t1=phase,
                          w0=np.ones(phase.size), w1=np.zeros(phase.size)))
     sino = torch.zeros(P.sino_shape, dtype=torch.float32, device=dev)
     fwd, bwd = _projection_times(P, truth, sino)
-    sino.zero_()
-    P.forward(truth, sino)
+    del sino
+    # analytic data of each view's phase (pixel-averaged line integrals in
+    # the projector's units, s_scl u_scl v_scl = 8 times the integral in
+    # voxel lengths, i.e. pitch^2 times the integral in mm), so 0.005 per
+    # unit is 0.02 mm^-1 per object unit at the 2 mm pitch
+    data = cone_projection_ellipsoids(geo, [phase_ells[p] for p in phase], device=dev)
     gen = torch.Generator(device=dev).manual_seed(4)
-    # the cone projector integrates over each detector pixel: its output is
-    # about pitch^2 times the line integral in mm, so 0.005 per output unit
-    # is 0.02 mm^-1 per object unit at the 2 mm pitch
-    y, w = transmission_noise(sino, photons, mu_scale, generator=gen)
+    y, w = transmission_noise(data, photons, mu_scale, generator=gen)
+    del data
     D = DataTerm(P, y, w)
     shape = (nt, 64, 128, 128)
     tv = TotalVariation(shape, tv_weight, tv_eps, wt=wt, samples=True, dtype=torch.float32,
@@ -376,7 +411,7 @@ def config4(iters=30, tv_weight=5.0, tv_eps=0.05, wt=1.0, photons=1e5, mu_scale=
                     _sampler(shape, torch.float32, dev))
     P.close()
     return dict(config=4, workload="4D cone, 10 phases x 128x128x64 x 2 mm, 256x192 detector 2 mm, "
-                "views per phase 30..60", views=int(phase.size), views_per_phase=counts.tolist(),
+                "views per phase 30..60, analytic ellipsoid data", views=int(phase.size), views_per_phase=counts.tolist(),
                 fwd_ms=1e3 * fwd, bwd_ms=1e3 * bwd, pairs_per_s=1.0 / (fwd + bwd), **sol)
 
 

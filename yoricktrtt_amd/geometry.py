@@ -343,3 +343,35 @@ def footprint_table(m: int, deg: int = 3):
         acc = acc + incr[i - 1]
         cum[i] = acc
     return fp, cum, coord, step
+
+
+def cone_rays(geom: Geometry, view: int, uv):
+    """Source and ray directions of a cone-beam view in voxel index
+    coordinates (i, j, k), derived from the view's own projective
+    coefficients (those of trtt_CB3D, trtt_3D_projectors.c:1190-1261): a point
+    p projects to uc = c3 - a4 num_u(p)/den(p), vc = b3 - a4 num_v(p)/den(p)
+    with num_u, num_v and den affine in p.  The source is their common zero;
+    the ray through detector point (u, v) (pixel-bound coordinates) is
+    orthogonal to the linear parts of num_u - a den and num_v - b den,
+    a = (u - c3)/(-a4), b = (v - b3)/(-a4).  ``uv`` is an (n, 2) array of
+    detector points; returns (source (3,), unit directions (n, 3))."""
+    xos = np.asarray(geom.xos[view], dtype=np.float64)
+    xsd = np.asarray(geom.xsd[view], dtype=np.float64)
+    a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12 = xsd
+    a1inv = -1.0 / a1
+    b1, b2, b3 = a6 + a1inv * a5 * a2, a7 + a1inv * a5 * a3, a8 + a1inv * a5 * a4
+    c1, c2, c3 = a10 + a1inv * a9 * a2, a11 + a1inv * a9 * a3, a12 + a1inv * a9 * a4
+    o = xos.reshape(3, 4)
+    lv = b1 * o[1, :3] + b2 * o[2, :3]
+    lu = c1 * o[1, :3] + c2 * o[2, :3]
+    ld = a1 * o[0, :3] + a2 * o[1, :3] + a3 * o[2, :3]
+    cv = b1 * o[1, 3] + b2 * o[2, 3]
+    cu = c1 * o[1, 3] + c2 * o[2, 3]
+    cd = a1 * o[0, 3] + a2 * o[1, 3] + a3 * o[2, 3]
+    src = np.linalg.solve(np.stack([lu, lv, ld]), -np.array([cu, cv, cd]))
+    uv = np.asarray(uv, dtype=np.float64)
+    al = (uv[:, 0] - c3) / (-a4)
+    be = (uv[:, 1] - b3) / (-a4)
+    d = np.cross(lu[None, :] - al[:, None] * ld[None, :], lv[None, :] - be[:, None] * ld[None, :])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return src, d

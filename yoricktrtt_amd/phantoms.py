@@ -47,17 +47,70 @@ def add_ellipsoid(vol, centre, axes, rot, value):
     return vol
 
 
-def ellipsoids_3d(shape, n_ell, seed, centre_frac=0.45, axes_frac=(0.06, 0.4), dtype=np.float32):
+def ellipsoids_3d(shape, n_ell, seed, centre_frac=0.45, axes_frac=(0.06, 0.4), dtype=np.float32,
+                  params=False):
     """n_ell additive ellipsoids: centres uniform in the central box of half
-    width centre_frac * n/2, semi-axes in axes_frac * n/2, values in [0, 1]."""
+    width centre_frac * n/2, semi-axes in axes_frac * n/2, values in [0, 1].
+    With ``params`` also the list of (centre, axes, rotation, value) in grid
+    units centred on the origin (for :func:`cone_projection_ellipsoids`)."""
     rng = np.random.default_rng(seed)
     vol = np.zeros(shape, dtype=dtype)
     half = np.array(shape[::-1], dtype=np.float64) / 2.0  # (x, y, z)
+    ells = []
     for _ in range(n_ell):
         c = rng.uniform(-centre_frac, centre_frac, size=3) * half
         a = rng.uniform(axes_frac[0], axes_frac[1], size=3) * half.min()
-        add_ellipsoid(vol, c, a, _rotation(rng), rng.uniform(0.0, 1.0))
-    return vol
+        rot, val = _rotation(rng), rng.uniform(0.0, 1.0)
+        add_ellipsoid(vol, c, a, rot, val)
+        ells.append((c, a, rot, val))
+    return (vol, ells) if params else vol
+
+
+def cone_projection_ellipsoids(geom, ells, views=None, device="cuda", quad=2):
+    """Analytic cone-beam data of a sum of ellipsoids: for every detector
+    pixel the line integral of the object along the ray from the source,
+    averaged over the pixel (quad x quad Gauss-Legendre points), scaled to
+    the footprint projector's units (s_scl u_scl v_scl times the integral in
+    voxel lengths, the measured normalisation of trtt_CB3D).  The chord of a
+    ray through an ellipsoid is that of Tomography_3D/Simul_Data/simul.c:
+    49-111 (a quadratic in the ray parameter after the ellipsoid's own
+    frame).  ``ells`` holds (centre, axes, rotation, value) rows in grid units
+    centred on the origin, or one such list per view (moving objects).
+    Returns [views][nv][nu] float32 on ``device``."""
+    import torch
+
+    from .geometry import cone_rays
+
+    views = range(geom.nviews) if views is None else views
+    half = np.array([(geom.nx - 1) / 2, (geom.ny - 1) / 2, (geom.nz - 1) / 2])
+    g, w = np.polynomial.legendre.leggauss(quad)
+    off = 0.5 + 0.5 * g  # points in the pixel [0, 1]
+    # (nv, nu, q, q) detector points, pixel-bound coordinates
+    V, U, QA, QB = np.meshgrid(np.arange(geom.nv), np.arange(geom.nu), off, off, indexing="ij")
+    uv = np.stack([(U + QA).ravel(), (V + QB).ravel()], axis=1)
+    wq = np.outer(w, w).ravel() / 4.0
+    scale = geom.s_scl * geom.u_scl * geom.v_scl
+    out = torch.zeros((len(views), geom.nv, geom.nu), dtype=torch.float32, device=device)
+    wq_t = torch.tensor(wq, dtype=torch.float64, device=device)
+    for n, v in enumerate(views):
+        src, d = cone_rays(geom, v, uv)
+        D = torch.tensor(d, dtype=torch.float64, device=device)
+        acc = torch.zeros(D.shape[0], dtype=torch.float64, device=device)
+        rows = ells[n] if ells and isinstance(ells[0], list) else ells
+        for c, a, rot, val in rows:
+            rot = np.asarray(rot, dtype=np.float64)
+            ainv = 1.0 / np.asarray(a, dtype=np.float64)
+            s0 = (rot.T @ (src - (np.asarray(c, dtype=np.float64) + half))) * ainv
+            M = torch.tensor((rot * ainv[None, :]).T, dtype=torch.float64, device=device)
+            Dp = D @ M.T
+            c2 = (Dp * Dp).sum(1)
+            c1 = Dp @ torch.tensor(s0, dtype=torch.float64, device=device)
+            c0 = float(s0 @ s0) - 1.0
+            det = c1 * c1 - c0 * c2
+            acc += val * 2.0 * torch.sqrt(torch.clamp(det, min=0.0)) / c2
+        pix = (acc.view(geom.nv, geom.nu, -1) * wq_t.view(1, 1, -1)).sum(-1)
+        out[n] = (scale * pix).float()
+    return out
 
 
 def body_phantom(n, seed, dtype=np.float32):
