@@ -193,6 +193,216 @@ def error_vs_fp64(geo, x_host, y_host, sino, vol, views=(0, 90, 180, 270), slab=
                       f"{NVIEWS} views on slices {slab[0]}..{slab[1] - 1}"}
 
 
+def _pool_time(fn, n, threads):
+    """Wall seconds of fn(i) for i < n over `threads` host threads (the
+    ctypes calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(fn, range(n)))
+    return time.perf_counter() - t0
+
+
+def _rel(a, b):
+    return float(f"{float(np.abs(a - b).max() / np.abs(b).max()):.3g}")
+
+
+def _gpu_pair(P, x, reps=5):
+    """Median forward and back projection ms of plan P on x (HBM-resident)."""
+    import torch
+
+    sino = torch.zeros(P.sino_shape, dtype=P.dtype, device=x.device)
+    vol = torch.zeros(P.vol_shape, dtype=P.dtype, device=x.device)
+    ts = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.forward(x, sino.zero_())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        P.backward(sino, vol.zero_())
+        torch.cuda.synchronize()
+        ts.append((t1 - t0, time.perf_counter() - t1))
+    f, b = np.median(np.array(ts[1:]), axis=0)
+    return 1e3 * f, 1e3 * b, sino, vol
+
+
+def _report_2d(cfg, geo, fan, prec, workload, ref, threads):
+    """GPU plan vs the reference C plugin on every view of a 2-D
+    configuration: both operators over all views on the host threads (the
+    full workload, timed), and the GPU results against those FP64 outputs."""
+    import torch
+
+    from yoricktrtt_amd.projectors import Projector
+
+    rng = np.random.default_rng(cfg)
+    x = rng.uniform(size=(geo.ny, geo.nx))
+    P = Projector(geo, model="cubic", precision=prec)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    fms, bms, sino, vol = _gpu_pair(P, torch.tensor(x, dtype=dt, device="cuda"))
+    ys = sino.double().cpu().numpy()   # the back projection's input, as on the GPU
+    back_gpu = vol.double().cpu().numpy()
+    P.close()
+    fn = ref.FB2D_cubic if fan else ref.PB2D_cubic
+    V = geo.nviews
+    fwd = np.zeros((V, geo.nv))
+    chunks = np.array_split(np.arange(V), threads)
+    parts = [np.zeros((geo.ny, geo.nx)) for _ in chunks]
+
+    def f(i):
+        fn(fwd[i], x, geo.nx, geo.ny, geo.s_scl, geo.nv, geo.v_scl, geo.xos[i], geo.xsd[i], 0)
+
+    def b(c):
+        for i in chunks[c]:
+            fn(parts[c], ys[i], geo.nx, geo.ny, geo.s_scl, geo.nv, geo.v_scl, geo.xos[i],
+               geo.xsd[i], 1)
+
+    tf = _pool_time(f, V, threads)
+    tb = _pool_time(b, len(chunks), threads)
+    back = np.sum(parts, axis=0)
+    # short operators: repeat to about 2 s of host time, keep the median
+    reps = int(min(20, max(0, 2.0 / (tf + tb) - 1)))
+    if reps:
+        fwd_keep = fwd.copy()
+        ts = [(tf, tb)]
+        for _ in range(reps):
+            for q in parts:
+                q[...] = 0.0
+            ts.append((_pool_time(f, V, threads), _pool_time(b, len(chunks), threads)))
+        tf, tb = np.median(np.array(ts), axis=0)
+        fwd = fwd_keep
+    return {"config": cfg, "workload": workload,
+            "gpu": {"fwd_ms": round(fms, 3), "bwd_ms": round(bms, 3),
+                    "pairs_per_s": round(1e3 / (fms + bms), 2), "dtype": prec},
+            "cpu_reference": {"pairs_per_s": round(1.0 / (tf + tb), 3), "wall_s": round(tf + tb, 3),
+                              "cores": threads, "kind": "reference",
+                              "sample": f"every view, forward and back ({'trtt_FB2D_cubic' if fan else 'trtt_PB2D_cubic'}, FP64), median of {reps + 1} runs"},
+            "gpu_over_cpu": round((tf + tb) * 1e3 / (fms + bms), 1),
+            "error_vs_fp64": {"forward_max_rel": _rel(ys, fwd), "backward_max_rel": _rel(back_gpu, back),
+                              "sample": "every view"}}
+
+
+def _report_lf(geo, prm, beta, threads):
+    """Configuration 2's Long & Fessler projector on the GPU against the
+    FP64 oracle over every view (the LF reference needs yapi.h and is not
+    built here: the oracle's OpenMP loop is the timed CPU port)."""
+    import torch
+
+    from oracle.oracle import LF2DParams, Oracle
+    from yoricktrtt_amd.projectors import LFProjector
+
+    rng = np.random.default_rng(22)
+    x = rng.uniform(size=(geo.ny, geo.nx))
+    lf = LFProjector(prm, beta, np.zeros(beta.size), precision="f32")
+    fms, bms, sino, vol = _gpu_pair(lf, torch.tensor(x, dtype=torch.float32, device="cuda"))
+    ys = sino.double().cpu().numpy()
+    back_gpu = vol.double().cpu().numpy()
+    lf.close()
+    o = Oracle()
+    po = LF2DParams(*(getattr(prm, f) for f, _ in prm._fields_))
+    t0 = time.perf_counter()
+    fwd = o.lf2d_views(po, beta, np.zeros(beta.size), x, 0)
+    t1 = time.perf_counter()
+    back = o.lf2d_views(po, beta, np.zeros(beta.size), ys, 1)
+    t2 = time.perf_counter()
+    return {"config": 2, "workload": "Long & Fessler SF projector on configuration 2's geometry",
+            "gpu": {"fwd_ms": round(fms, 3), "bwd_ms": round(bms, 3),
+                    "pairs_per_s": round(1e3 / (fms + bms), 2), "dtype": "f32"},
+            "cpu_reference": {"pairs_per_s": round(1.0 / (t2 - t0), 3), "wall_s": round(t2 - t0, 2),
+                              "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+                              "kind": "port",
+                              "sample": "every view, forward and back (oracle lf2d loop, FP64, OpenMP over views)"},
+            "gpu_over_cpu": round((t2 - t0) * 1e3 / (fms + bms), 1),
+            "error_vs_fp64": {"forward_max_rel": _rel(ys, fwd), "backward_max_rel": _rel(back_gpu, back),
+                              "sample": "every view"}}
+
+
+def _report_4d(ref, threads):
+    """Configuration 4 (10 phases, 516 views): the GPU plan over every view;
+    the reference on one view pair per host thread (forward of the view's
+    phase, back projection of its row), scaled to 516 views; the GPU
+    against those FP64 outputs (a plan over the sampled views for the back
+    projection)."""
+    import torch
+
+    from yoricktrtt_amd.configs import _tuned, config4_setup
+    from yoricktrtt_amd.projectors import Projector
+
+    geo, phase, _ = config4_setup()
+    nt = int(phase.max()) + 1
+    one = np.ones(phase.size)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(size=(nt, geo.nz, geo.ny, geo.nx))
+    xd = torch.tensor(x, dtype=torch.float32, device="cuda")
+    P = _tuned(Projector(geo, model="cubic", precision="f32", nt=nt, t0=phase, t1=phase, w0=one,
+                         w1=np.zeros(phase.size)))
+    fms, bms, sino, _ = _gpu_pair(P, xd)
+    P.close()
+    views = np.linspace(0, phase.size - 1, threads).astype(int)
+    ys = sino[views].double().cpu().numpy()
+    sino_gpu = ys.copy()
+    fwd = np.zeros((views.size, geo.nv, geo.nu))
+    bk = np.zeros((views.size, geo.nz, geo.ny, geo.nx))
+
+    def pair(j):
+        v = views[j]
+        ref.CB3D_cubic(fwd[j], x[phase[v]], geo.nx, geo.ny, geo.nz, geo.s_scl, geo.nu, geo.nv,
+                       geo.u_scl, geo.v_scl, geo.xos[v], geo.xsd[v], 0)
+        ref.CB3D_cubic(bk[j], ys[j], geo.nx, geo.ny, geo.nz, geo.s_scl, geo.nu, geo.nv, geo.u_scl,
+                       geo.v_scl, geo.xos[v], geo.xsd[v], 1)
+
+    wall = _pool_time(pair, views.size, threads)
+    back = np.zeros_like(x)
+    for j, v in enumerate(views):
+        back[phase[v]] += bk[j]
+    sub = geo.subset(views)
+    Ps = Projector(sub, model="cubic", precision="f32", nt=nt, t0=phase[views], t1=phase[views],
+                   w0=np.ones(views.size), w1=np.zeros(views.size))
+    vol = Ps.backward(torch.tensor(ys, dtype=torch.float32, device="cuda"))
+    back_gpu = vol.double().cpu().numpy()
+    Ps.close()
+    rate = views.size / wall / phase.size
+    return {"config": 4, "workload": "4-D cone, 10 phases x 128x128x64 x 2 mm, 516 views, 256x192 "
+                                     "detector 2 mm, Rsc 1000, Rsd 1500",
+            "gpu": {"fwd_ms": round(fms, 3), "bwd_ms": round(bms, 3),
+                    "pairs_per_s": round(1e3 / (fms + bms), 2), "dtype": "f32"},
+            "cpu_reference": {"pairs_per_s": round(rate, 4), "wall_s": round(wall, 2),
+                              "cores": threads, "kind": "reference",
+                              "sample": f"{views.size} threads x (1 forward + 1 back view of "
+                                        "trtt_CB3D_cubic, FP64), scaled to 516 views"},
+            "gpu_over_cpu": round(1e3 / (fms + bms) / rate, 1),
+            "error_vs_fp64": {"forward_max_rel": _rel(sino_gpu, fwd),
+                              "backward_max_rel": _rel(back_gpu, back),
+                              "sample": f"the {views.size} timed views (forward of the full plan; "
+                                        "back projection by a plan over those views)"}}
+
+
+def configs_report(threads):
+    """Configurations 1, 2 (and its Long & Fessler projector) and 4 beside
+    the reference CPU path: GPU projection times, the reference timed on the
+    host cores, and the GPU's max error against the FP64 results."""
+    from oracle.oracle import REF_O3_SO, REF_SO, Reference
+    from yoricktrtt_amd.configs import config1_setup, config2_setup
+
+    path = REF_O3_SO if Reference.available(REF_O3_SO) else REF_SO
+    ref = Reference(path)
+    build = "O3_x86-64-v3" if path == REF_O3_SO else "O2"
+    out = []
+    g1 = config1_setup()
+    out.append(_report_2d(1, g1, False, "f64", "2-D parallel 256^2, 360 views / 180 deg, 367 bins",
+                          ref, threads))
+    g2, prm, beta = config2_setup()
+    out.append(_report_2d(2, g2, True, "f32", "2-D fan 512^2 x 0.5 mm, 720 views / 360 deg, 736 "
+                          "flat bins 0.75 mm, Rsc 1000, Rsd 1500", ref, threads))
+    out.append(_report_lf(g2, prm, beta, threads))
+    out.append(_report_4d(ref, threads))
+    for r in out:
+        if r["cpu_reference"]["kind"] == "reference":
+            r["cpu_reference"]["build"] = build
+    return out
+
+
 def table_model_rate(x, y, steps=3):
     """Throughput of the tabulated footprint (the model the Yorick scripts
     call, Ytrtt_3D_projectors.i:124-128; 10^5-entry table as
@@ -288,6 +498,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the table-model and host-array (PCIe) figures")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
+    ap.add_argument("--configs-report", default=None, metavar="PATH",
+                    help="after the benchmark line, write configurations 1, 2 (with Long & "
+                         "Fessler) and 4 beside the reference CPU path to PATH (JSON lines)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run an elementwise copy of 1 GiB (HBM counter calibration)")
     ap.add_argument("--tune-cache", default=os.path.join(HERE, "yoricktrtt_amd", "tuning",
@@ -441,6 +654,12 @@ def main():
         result["cpu_baseline"] = None
     P.close()
     print(json.dumps(result), flush=True)
+    if args.configs_report:
+        print("bench: configurations 1, 2, 4 beside the reference", file=sys.stderr, flush=True)
+        with open(args.configs_report, "w") as f:
+            for r in configs_report(args.cpu_threads):
+                print("bench: " + json.dumps(r), file=sys.stderr, flush=True)
+                f.write(json.dumps(r) + "\n")
 
 
 if __name__ == "__main__":

@@ -260,3 +260,11 @@ class Reference:
     def PB3D_cubic(self, out, inp, nx, ny, nz, s, nu, nv, us, vs, xos, xsd, job, deg=3):
         return self.lib.trtt_PB3D_cubic(_p(out), _p(_f64(inp)), nx, ny, nz, s, deg, nu, nv, us, vs,
                                         _p(_f64(xos)), _p(_f64(xsd)), job)
+
+    def PB2D_cubic(self, out, inp, nx, ny, s, nv, vs, xos, xsd, job, deg=3):
+        return self.lib.trtt_PB2D_cubic(_p(out), _p(_f64(inp)), nx, ny, s, deg, nv, vs,
+                                        _p(_f64(xos)), _p(_f64(xsd)), job)
+
+    def FB2D_cubic(self, out, inp, nx, ny, s, nv, vs, xos, xsd, job, deg=3):
+        return self.lib.trtt_FB2D_cubic(_p(out), _p(_f64(inp)), nx, ny, s, deg, nv, vs,
+                                        _p(_f64(xos)), _p(_f64(xsd)), job)
