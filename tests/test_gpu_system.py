@@ -1,7 +1,9 @@
 """GPU projector set of a 4-D system after a save/load round trip
 (yoricktrtt_amd.system): the loaded system's C_k = A_k B_k forward and
-transpose equal the original system's bit for bit and the FP64 oracle view
-by view on the linearly interpolated phases (1e-12 relative)."""
+transpose equal the original system's (1e-14 relative: the run-to-run
+spread of the atomic FP64 kernels; the plan inputs themselves are equal bit
+for bit, checked on the CPU) and the FP64 oracle view by view on the
+linearly interpolated phases (1e-12 relative)."""
 import math
 
 import numpy as np
@@ -45,7 +47,9 @@ def test_loaded_system_projectors(lib, oracle, tmp_path):
         torch.cuda.synchronize()
         outs.append((sino.cpu().numpy(), back.cpu().numpy()))
         P.close()
-    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    # the same plan inputs; the general FP64 kernels accumulate with
+    # atomics, so two runs agree to their run-to-run spread, not bit for bit
+    assert rel_err(outs[1][0], outs[0][0]) < 1e-14 and rel_err(outs[1][1], outs[0][1]) < 1e-14
     g = lo.geometry()
     idx, w = temporal_weights(lo.t_index(), nt, 1)
     ref = np.zeros((V, nv, nu))
