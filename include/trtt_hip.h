@@ -291,6 +291,10 @@ typedef struct trtt_hip_tuning {
   int fwd_views_per_block; /* views per forward workgroup, 0 = from the grid size */
   int bwd_window_bytes;    /* LDS detector window of a backward workgroup, 4096..131072 */
   int fwd_chunk_views;     /* views per forward scratch round, 0 = as many as fit 4 GiB */
+  int fwd_flush;           /* 1 (default): each tile's LDS window is added to the
+                              projections with global atomic adds; 0: windows go to
+                              scratch slots and a reduction kernel sums them (on a
+                              second stream, overlapped with the next chunk) */
 } trtt_hip_tuning_t;
 int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *plan, trtt_hip_tuning_t *tuning);
 int trtt_hip_plan_set_tuning(trtt_hip_plan_t *plan, const trtt_hip_tuning_t *tuning);

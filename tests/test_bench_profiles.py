@@ -39,8 +39,10 @@ def test_profiles_ignored_for_another_launch_shape():
     assert bench.measured_issue(None, "axial::bwd", 10.0) is None
 
 
-def test_forward_traffic_reads_the_volume_once_per_view():
+def test_forward_traffic_within_the_per_view_bytes():
     vals = bench.profile_values(bench.TRAFFIC_FILE, shipped_tuning())
-    # FETCH_SIZE of the forward: the volume once per view of the launch, within 5 %
-    assert vals["fwd_fetch_bytes_per_launch"] == pytest.approx(
-        vals["fwd_algorithmic_bytes_per_launch"], rel=0.05)
+    # the windows go to the projections with atomics (no scratch round trip):
+    # fetched plus written bytes of the forward stay below one volume read
+    # and one projection write per view
+    assert vals["fwd_launches"] == 1
+    assert 0 < vals["fwd_traffic_bytes_per_launch"] <= vals["fwd_algorithmic_bytes_per_launch"]

@@ -12,8 +12,8 @@ compiled from its sources when ``oracle/_ref`` is present:
      launch shape and the default one; forward on 12 evenly spaced views of
      the 360-view plan, backward of all 360 views on four 4-slice slabs
      (the FP32 register accumulators run over all 360 views), the full
-     360-view adjoint identity in FP64 sums, 1e-5 / 1e-6; the table model
-     (general kernels) on the same views and slabs, 1e-5;
+     360-view adjoint identity in FP64 sums, 1e-5 / 1e-6; the tabulated
+     footprint on the same views and slabs, 1e-5;
   4  4-D cone, 10 phases x 128x128x64 x 2 mm, 516 views (30-60 per phase),
      256 x 192 at 2 mm: cost and gradient of the weighted data term over all
      phases against the oracle loop, eager and graph-replayed, 1e-5.
@@ -21,7 +21,9 @@ compiled from its sources when ``oracle/_ref`` is present:
 Parity measure: max |gpu - ref| / max |ref| per output.
 """
 import ctypes as C
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -33,8 +35,9 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 # launch shape shipped in yoricktrtt_amd/tuning/tuning.json for configuration 3
-SHIPPED3 = dict(fwd_tile=1, fwd_window_bytes=8192, fwd_views_per_block=3, fwd_chunk_views=0,
-                bwd_tile=3, bwd_window_bytes=20480)
+_TUNING = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..",
+                                      "yoricktrtt_amd", "tuning", "tuning.json")))
+SHIPPED3 = next(v["tuning"] for k, v in _TUNING.items() if "cone:256x256x256x1:384x512:360" in k)
 SLABS3 = [(0, 4), (84, 88), (170, 174), (252, 256)]
 
 

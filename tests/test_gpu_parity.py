@@ -814,3 +814,54 @@ def test_atomic_forwards_run_to_run(lib, case):
     _say(f"  run-to-run {worst:.2e}")
     assert worst < tol
     P.close()
+
+
+@pytest.mark.parametrize("nt", [1, 3])
+@pytest.mark.parametrize("model", ["cubic", "table"])
+def test_axial_forward_tiles_vs_oracle(lib, oracle, nt, model):
+    """Every forward launch shape with several views per workgroup, both
+    ways of adding the tile windows to the projections (global atomics, or
+    scratch slots summed by the reduction kernel), against the FP64 oracle:
+    a slice count that leaves a partial last tile, a detector smaller than
+    the projected volume, views interpolated between two phases (nt = 3),
+    and the smallest window (most tiles take the direct path)."""
+    from yoricktrtt_amd import autotune as AT
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    nx, ny, nz, nu, nv, V = 21, 18, 150, 230, 36, 7
+    _say("forward tiles", nt, model)
+    rng = np.random.default_rng(60 + nt)
+    th = rng.uniform(0, 2 * math.pi, size=V)
+    geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, th, 300.0, 450.0, v_off=1.5)
+    t0 = np.zeros(V, np.int32)
+    t1 = np.zeros(V, np.int32)
+    w0 = np.ones(V)
+    w1 = np.zeros(V)
+    kw = {}
+    if nt > 1:
+        t = rng.uniform(0, nt - 1, size=V)
+        t[:3] = np.floor(t[:3])
+        t0 = np.floor(t).astype(np.int32)
+        t1 = np.minimum(t0 + 1, nt - 1).astype(np.int32)
+        w1 = t - t0
+        w1[t0 == t1] = 0.0
+        w0 = 1.0 - w1
+        kw = dict(nt=nt, t0=t0, t1=t1, w0=w0, w1=w1)
+    P = Projector(geo, model=model, precision="f32", **kw)
+    assert P.path > 0
+    x = rng.uniform(size=(nt, nz, ny, nx))
+    rf = np.zeros((V, nv, nu))
+    for v in range(V):
+        xv = w0[v] * x[t0[v]] + w1[v] * x[t1[v]]
+        oracle.sb3d(rf[v], xv, nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, geo.xos[v], geo.xsd[v], True, 0,
+                    _tab(model))
+    xd = _dev(x.reshape(P.vol_shape), torch.float32)
+    for tile in sorted(AT.FWD_TILES):
+        for wb, vpb, flush in [(24576, 0, 1), (16384, 3, 0), (16384, 3, 1), (4096, 1, 1)]:
+            AT.set_tuning(P, fwd_tile=tile, fwd_window_bytes=wb, fwd_views_per_block=vpb,
+                          fwd_flush=flush)
+            fwd = P.forward(xd).double().cpu().numpy()
+            assert rel_err(fwd, rf) < 1e-5, (tile, wb, vpb, flush)
+    assert P.error_flag() == 0
+    P.close()

@@ -244,7 +244,7 @@ def test_autotune_candidates_and_cache(tmp_path):
     cache = str(tmp_path / "tuning.json")
     r = AT.autotune(P, cache=cache, reps=1, windows=(8192, 16384), vpbs=(0, 2), chunks=(0, 3))
     assert not r["cached"] and AT.get_tuning(P) == r["tuning"]
-    assert len(r["rows"]) == 3 * 2 * 2 * 2 + sum(len(w) for w in AT.BWD_WINDOWS.values())
+    assert len(r["rows"]) == len(AT.FWD_TILES) * 2 * 2 * (1 + 2) + sum(len(w) for w in AT.BWD_WINDOWS.values())
     r2 = AT.autotune(P, cache=cache)
     assert r2["cached"] and r2["tuning"] == r["tuning"]
     with pytest.raises(RuntimeError):
@@ -276,7 +276,8 @@ def test_overlapped_reduction_eager_and_graph():
     x = torch.tensor(rng.uniform(size=(nt, geo.nz, geo.ny, geo.nx)), dtype=torch.float32,
                      device="cuda")
     one = P.forward(x).clone()
-    AT.set_tuning(P, fwd_chunk_views=2)  # 6 chunks, two buffers, two streams
+    # scratch windows and the reduction kernel: 6 chunks, two buffers, two streams
+    AT.set_tuning(P, fwd_chunk_views=2, fwd_flush=0)
     for _ in range(3):
         many = P.forward(x)
         assert rel_err(many.cpu().numpy(), one.cpu().numpy()) < 1e-6

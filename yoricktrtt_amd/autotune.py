@@ -61,11 +61,16 @@ def set_tuning(P, **kw):
 def candidates(windows=WINDOWS, vpbs=VPBS, chunks=None):
     """Forward and backward candidates, tuned independently (the two
     kernels share no launch parameter).  ``windows`` maps a forward tile to
-    its window sizes, or is one tuple for every tile."""
+    its window sizes, or is one tuple for every tile.  Forward windows are
+    added to the projections with atomics (one round over all views), or
+    go through scratch and the reduction kernel in rounds of ``chunks``
+    views."""
     per_tile = windows if isinstance(windows, dict) else {t: windows for t in FWD_TILES}
     chunks = CHUNKS if chunks is None else chunks
-    fwd = [dict(fwd_tile=t, fwd_window_bytes=wb, fwd_views_per_block=v, fwd_chunk_views=c)
-           for t in FWD_TILES for wb, v, c in itertools.product(per_tile[t], vpbs, chunks)]
+    modes = [(1, 0)] + [(0, c) for c in chunks]
+    fwd = [dict(fwd_tile=t, fwd_window_bytes=wb, fwd_views_per_block=v, fwd_flush=f,
+                fwd_chunk_views=c)
+           for t in FWD_TILES for wb, v, (f, c) in itertools.product(per_tile[t], vpbs, modes)]
     bwd = [dict(bwd_tile=t, bwd_window_bytes=wb) for t in BWD_TILES for wb in BWD_WINDOWS[t]]
     return fwd, bwd
 
@@ -144,7 +149,11 @@ def autotune(P, x=None, y=None, cache=DEFAULT_CACHE, reps=3, windows=WINDOWS, vp
     best_f, best_b = None, None
     rows = []
     for c in fwd_c:
-        set_tuning(P, **c)
+        try:
+            set_tuning(P, **c)
+        except RuntimeError:  # a tile the plan's geometry does not allow
+            rows.append(dict(kind="fwd", **c, ms=None, rejected=True))
+            continue
         t1 = once(lambda: P.forward(x, sino.zero_()))
         err = float((sino - ref_f).abs().max()) / scale_f
         if err > 1e-5:

@@ -746,7 +746,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
                                           T *__restrict__ sino, int views_per_block, int wcap,
                                           int pmax, int view_base, int view_end,
                                           T *__restrict__ scratch, int4 *__restrict__ hdr,
-                                          int plain) {
+                                          int plain, int flush) {
   constexpr int NCOL = CX * CY;  // columns of the wave
   static_assert(NCOL <= 64, "one column per lane");
   // The previous chunk's reduction runs on a second stream beside this
@@ -897,12 +897,23 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         }
       }
     }
-    if (in_lds) {
+    if (in_lds && flush) {
+      // add the window to the view's projection: global atomic adds (in
+      // L2), the lanes on consecutive pixels of a row; pixels past the
+      // detector's last column (the rounded-up row end) are skipped
+      for (int r = 0; r < RH; ++r) {
+        T *orow = out + long(vl + r) * P.nu;
+        for (int cI = lane; cI < RW; cI += 64) {
+          const T wv = win[r * RW + cI];
+          if (ul + cI < P.nu && wv != T(0)) unsafeAtomicAdd(orow + ul + cI, wv);
+        }
+      }
+    } else if (in_lds) {
       V16 *dst = reinterpret_cast<V16 *>(scratch + slot * wcap);
       const V16 *w16 = reinterpret_cast<const V16 *>(win);
       for (int q = lane; q < nvec; q += 64) dst[q] = w16[q];
     }
-    if (lane == 0) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
+    if (lane == 0 && !flush) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -1244,7 +1255,8 @@ int axial_tiles(const DevPlan &P, int cx, int cy, int kt) {
 }
 
 int axial_tiles(const DevPlan &P, int job, const AxialTuning &tu) {
-  if (job == 0) return axial_tiles(P, kFwdTiles[tu.fwd_tile][0], kFwdTiles[tu.fwd_tile][1], axial::kKT);
+  if (job == 0)
+    return axial_tiles(P, kFwdTiles[tu.fwd_tile][0], kFwdTiles[tu.fwd_tile][1], axial::kKT);
   return axial_tiles(P, kBwdTiles[tu.bwd_tile][0], kBwdTiles[tu.bwd_tile][1],
                      axial::kKT * bwd_tile_vpl(tu.bwd_tile));
 }
@@ -1269,13 +1281,14 @@ size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
 
 template <bool CONE, int W, int CX, int CY, typename T>
 void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *volk, T *out,
-                int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain) {
+                int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain,
+                int flush) {
   if (P.table)
     hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, true, T>), grid, dim3(64), lds_bytes, st, P,
-                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
+                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush);
   else
     hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, false, T>), grid, dim3(64), lds_bytes, st, P,
-                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain);
+                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush);
 }
 
 template <bool CONE, int W, int CX, int CY, int VPL, typename T>
@@ -1312,7 +1325,8 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
                        dim3(axial::kThreads), 0, st, static_cast<const T *>(in), volk, P.nx, P.ny,
                        P.nz);
     const int nchunks = (P.nviews + chunk - 1) / chunk;
-    const bool overlap = ov && ov->aux && nbuf == 2 && nchunks > 1;
+    const int flush = tu.fwd_flush;
+    const bool overlap = !flush && ov && ov->aux && nbuf == 2 && nchunks > 1;
     const size_t win_elems = size_t(chunk) * ntiles * size_t(wcap);
     const size_t hdr_bytes = axial_hdr_bytes(P, chunk, tu);
     for (int c = 0, vb = 0; vb < P.nviews; vb += chunk, ++c) {
@@ -1328,18 +1342,19 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
       switch (tu.fwd_tile) {
         case 1:
           launch_fwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain);
+                                       vb, ve, sc, hd, plain, flush);
           break;
         case 2:
           launch_fwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain);
+                                       vb, ve, sc, hd, plain, flush);
           break;
         default:
           launch_fwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain);
+                                       vb, ve, sc, hd, plain, flush);
       }
       if (ev) hipEventRecord(ev[2 * npair + 1], st);
       ++npair;
+      if (flush) continue;  // the windows are in the projections already
       hipStream_t rs = st;
       if (overlap) {
         hipEventRecord(ov->fwd_done[b], st);

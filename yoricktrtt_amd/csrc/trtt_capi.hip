@@ -613,12 +613,28 @@ trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, lon
   return p;
 }
 
-// Scratch of the axial forward: windows of `chunk` views at a time, at most
-// 4 GiB (views are processed in chunks that fit).
+// k-fastest copy of the volume series read by the axial forward.
+bool alloc_volk(trtt_hip_plan *p) {
+  if (!p->d_volk) {
+    const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
+    if (!hip_ok(hipMalloc(&p->d_volk, vbytes), "hipMalloc scratch")) return false;
+  }
+  return true;
+}
+
+// Scratch of the axial forward: none when the windows are flushed with
+// atomics; else windows of `chunk` views at a time, at most 4 GiB (views
+// are processed in chunks that fit).
 bool alloc_axial_scratch(trtt_hip_plan *p) {
   hipFree(p->d_scratch);
   hipFree(p->d_hdr);
   p->d_scratch = p->d_hdr = nullptr;
+  if (p->tune.fwd_flush) {
+    // windows go straight to the projections: no scratch, one round
+    p->axial_chunk = int(p->nviews);
+    p->axial_nbuf = 1;
+    return alloc_volk(p);
+  }
   const size_t per_view = trtt::axial_scratch_bytes(p->dev, 1, p->tune);
   const size_t budget = size_t(4) << 30;
   long chunk = long(budget / per_view);
@@ -643,12 +659,7 @@ bool alloc_axial_scratch(trtt_hip_plan *p) {
                   "hipEventCreate"))
         return false;
   }
-  // k-fastest copy of the volume series read by the forward
-  if (!p->d_volk) {
-    const size_t vbytes = size_t(p->nt) * p->nx * p->ny * p->nz * sizeof(float);
-    if (!hip_ok(hipMalloc(&p->d_volk, vbytes), "hipMalloc scratch")) return false;
-  }
-  return true;
+  return alloc_volk(p);
 }
 
 // One launch of a plan (forward job 0 / backward job 1).
@@ -826,6 +837,7 @@ int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *p, trtt_hip_tuning_t *t) {
   t->fwd_views_per_block = p->tune.fwd_views_per_block;
   t->bwd_window_bytes = p->tune.bwd_window_bytes;
   t->fwd_chunk_views = p->tune.fwd_chunk_views;
+  t->fwd_flush = p->tune.fwd_flush;
   return TRTT_SUCCESS;
 }
 
@@ -841,6 +853,7 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
   if (t->bwd_window_bytes < 4096 || t->bwd_window_bytes > 131072 || t->bwd_window_bytes % 256)
     return fail("backward window must be 4096..131072 bytes, a multiple of 256");
   if (t->fwd_chunk_views < 0) return fail("views per forward chunk must be >= 0");
+  if (t->fwd_flush != 0 && t->fwd_flush != 1) return fail("forward flush mode must be 0 or 1");
   if (p->axial_w <= 0) return TRTT_SUCCESS;  // general kernels: nothing to tune
   if (trtt::bwd_tile_vpl(t->bwd_tile) == 2 && p->axial_pmax2 == 0)
     return fail("backward tiles of 128 slices need column profiles of at most 256 pixels");
@@ -853,8 +866,9 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
   p->tune.fwd_views_per_block = t->fwd_views_per_block;
   p->tune.bwd_window_bytes = t->bwd_window_bytes;
   p->tune.fwd_chunk_views = t->fwd_chunk_views;
+  p->tune.fwd_flush = t->fwd_flush;
   if (old.fwd_tile != p->tune.fwd_tile || old.fwd_window_bytes != p->tune.fwd_window_bytes ||
-      old.fwd_chunk_views != p->tune.fwd_chunk_views) {
+      old.fwd_chunk_views != p->tune.fwd_chunk_views || old.fwd_flush != p->tune.fwd_flush) {
     // hipFree waits for the device: no launch of this plan is in flight
     if (!alloc_axial_scratch(p)) {
       p->tune = old;

@@ -101,6 +101,8 @@ struct AxialTuning {
   int fwd_views_per_block = 0;    // 0: from the grid size
   int bwd_window_bytes = 32768;   // shared LDS window of a backward workgroup
   int fwd_chunk_views = 0;        // views per forward scratch round, 0: fill the budget
+  int fwd_flush = 1;              // 1: windows added to the projections with global atomics;
+                                  // 0: written to scratch slots, summed by fwd_reduce
 };
 
 // Second stream of the axial forward: the reduction of view chunk c runs on
