@@ -454,15 +454,20 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
 // m > 1 where voxels m slices apart do, so the lanes of one residue of k
 // mod m run the plain steps together, one residue after the other; 0:
 // LDS atomics.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <int NP, bool TAB, typename T>
 __device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v, T *pp, T val,
                                               int passes) {
   T w[NP];
   uweights<NP, TAB>(P, v, w);
+  // plain steps: one FMA per pixel into the profile entry (the product is
+  // not formed apart, so it is not shared with the atomic branch)
   if (passes == 1) {
 #pragma unroll
     for (int l = 0; l < NP; ++l) {
-      pp[l] += w[l] * val;
+      pp[l] = fma_t(w[l], val, pp[l]);
       __builtin_amdgcn_wave_barrier();
     }
   } else if (passes > 1) {
@@ -471,7 +476,7 @@ __device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v,
       if (r == ps) {
 #pragma unroll
         for (int l = 0; l < NP; ++l) {
-          pp[l] += w[l] * val;
+          pp[l] = fma_t(w[l], val, pp[l]);
           __builtin_amdgcn_wave_barrier();
         }
       }
@@ -899,14 +904,24 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     }
     if (in_lds && flush) {
       // add the window to the view's projection: global atomic adds (in
-      // L2), the lanes on consecutive pixels of a row; pixels past the
-      // detector's last column (the rounded-up row end) are skipped
-      for (int r = 0; r < RH; ++r) {
-        T *orow = out + long(vl + r) * P.nu;
-        for (int cI = lane; cI < RW; cI += 64) {
-          const T wv = win[r * RW + cI];
-          if (ul + cI < P.nu && wv != T(0)) unsafeAtomicAdd(orow + ul + cI, wv);
+      // L2), the lanes on consecutive pixels of a row, zeros skipped; the
+      // columns past the detector's last pixel (the rounded-up row end)
+      // are not added
+      const int cw = min(RW, P.nu - ul);
+      T *orow = out + long(vl) * P.nu + ul;
+      if (cw <= 128) {
+        const bool c0 = lane < cw, c1 = lane + 64 < cw;
+        for (int r = 0; r < RH; ++r, orow += P.nu) {
+          const T a = win[r * RW + lane], b = win[r * RW + lane + 64];
+          if (c0 && a != T(0)) unsafeAtomicAdd(orow + lane, a);
+          if (c1 && b != T(0)) unsafeAtomicAdd(orow + lane + 64, b);
         }
+      } else {
+        for (int r = 0; r < RH; ++r, orow += P.nu)
+          for (int cI = lane; cI < cw; cI += 64) {
+            const T a = win[r * RW + cI];
+            if (a != T(0)) unsafeAtomicAdd(orow + cI, a);
+          }
       }
     } else if (in_lds) {
       V16 *dst = reinterpret_cast<V16 *>(scratch + slot * wcap);
