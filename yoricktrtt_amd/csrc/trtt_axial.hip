@@ -754,10 +754,11 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
                                           int plain, int flush) {
   constexpr int NCOL = CX * CY;  // columns of the wave
   static_assert(NCOL <= 64, "one column per lane");
-  // The previous chunk's reduction runs on a second stream beside this
-  // kernel: this kernel is bound by instruction issue and the reduction by
-  // memory latency, so these waves issue first (forward 43.6 -> 42.6 ms on
-  // the 256^3 cone benchmark; priorities 1, 2 and 3 measure the same).
+  // With scratch windows (flush 0) the previous chunk's reduction runs on
+  // a second stream beside this kernel: this kernel is bound by instruction
+  // issue and the reduction by memory latency, so these waves issue first
+  // (forward 43.6 -> 42.6 ms on the 256^3 cone benchmark; priorities 1, 2
+  // and 3 measure the same).
   __builtin_amdgcn_s_setprio(1);
   const int lane = threadIdx.x;
   T *win = reinterpret_cast<T *>(g_smem);
