@@ -60,3 +60,31 @@ def test_out_of_range_time_is_refused():
         temporal_weights([-0.1], 5, 1)
     with pytest.raises(ValueError):
         temporal_weights([1.0], 5, 4)
+
+
+def test_data_term_converts_and_checks_its_data():
+    """DataTerm stores y and w in the plan's type and refuses another shape
+    (the residual kernel reads them as the plan's type and shape); the
+    projector's own checks refuse host tensors and a non-FP64 fx."""
+    import torch
+
+    from yoricktrtt_amd.projectors import Projector
+    from yoricktrtt_amd.reconstruction import DataTerm
+
+    class Stub:
+        sino_shape = (3, 4, 5)
+        dtype = torch.float32
+
+    D = DataTerm(Stub(), torch.ones((3, 4, 5), dtype=torch.float64),
+                 torch.ones((3, 4, 5), dtype=torch.float64))
+    assert D.y.dtype == torch.float32 and D.w.dtype == torch.float32
+    with pytest.raises(ValueError, match="shape"):
+        DataTerm(Stub(), torch.ones((3, 4, 6)))
+    with pytest.raises(ValueError, match="shape"):
+        DataTerm(Stub(), torch.ones((3, 4, 5)), torch.ones((3, 5, 4)))
+    P = Projector.__new__(Projector)  # argument checks only, no plan
+    P.torch, P.dtype = torch, torch.float32
+    with pytest.raises(TypeError):
+        P._check(torch.ones((3, 4, 5)), (3, 4, 5))
+    with pytest.raises(ValueError, match="fx"):
+        P._check_fx(torch.zeros(1, dtype=torch.float32))

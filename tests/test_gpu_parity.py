@@ -865,3 +865,27 @@ def test_axial_forward_tiles_vs_oracle(lib, oracle, nt, model):
             assert rel_err(fwd, rf) < 1e-5, (tile, wb, vpb, flush)
     assert P.error_flag() == 0
     P.close()
+
+
+def test_weighted_residual_refuses_mismatched_arguments(lib):
+    """Data of another type or shape, and an fx that is not one FP64
+    element, are refused before any launch (the kernel would read them as
+    the plan's type and shape)."""
+    from yoricktrtt_amd.geometry import cone_geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("residual argument checks")
+    geo = cone_geometry(8, 8, 8, 1.0, 10, 12, 1.0, 1.0, [0.0, 1.0], 50.0, 80.0)
+    P = Projector(geo, precision="f32")
+    s = torch.zeros(P.sino_shape, device="cuda:0")
+    fx = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(ValueError):
+        P.weighted_residual(s, torch.zeros(P.sino_shape, dtype=torch.float64, device="cuda:0"),
+                            None, fx)
+    with pytest.raises(ValueError):
+        P.weighted_residual(s, s.clone(), torch.zeros((2, 12, 9), device="cuda:0"), fx)
+    with pytest.raises(ValueError):
+        P.weighted_residual(s, s.clone(), None, torch.zeros(1, device="cuda:0"))
+    with pytest.raises(ValueError):
+        P.weighted_residual(s, s.clone(), None, torch.zeros(2, dtype=torch.float64, device="cuda:0"))
+    P.close()
