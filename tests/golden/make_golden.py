@@ -14,7 +14,8 @@ and records inputs and outputs of every projector on small seeded cases:
                     smaller and larger than the voxels
 * ``deg2.npz``      trtt_{PB,CB}3D and trtt_{PB,FB}2D with s_deg = 2 and a
                     degree-2 table (10^3 and 10^4 entries) on the geometries
-                    and inputs of proj3d / proj2d
+                    and inputs of proj3d / proj2d, and the ``_cubic`` forms
+                    called with s_deg = 2 (keys ``cubic/...``)
 
 The per-view XOS/XSD pairs are composed with the reference transform code in
 the same sequence as the Yorick system builders
@@ -258,6 +259,49 @@ def deg2_cases(lib, proj3d, proj2d):
                     assert r == 0
             out[f"m{m}/2d/{name}/fwd"] = fwd
             out[f"m{m}/2d/{name}/bwd"] = bwd
+    # the closed-form cubic symbols called with s_deg = 2: the cubic CDF on
+    # the pixel window of a 1.5-voxel half support (a truncated footprint)
+    vol = np.ascontiguousarray(proj3d["vol"])
+    nz, ny, nx = vol.shape
+    s = float(proj3d["s"])
+    for name in ("cone_a", "cone_edge", "par_a"):
+        nu, nv = (int(v) for v in proj3d[f"{name}/dims"])
+        us, vs = (float(v) for v in proj3d[f"{name}/scl"])
+        xos, xsd = proj3d[f"{name}/xos"], proj3d[f"{name}/xsd"]
+        sino_in = proj3d[f"{name}/sino_in"]
+        nview = xos.shape[0]
+        fn = getattr(lib, "trtt_CB3D_cubic" if name.startswith("cone") else "trtt_PB3D_cubic")
+        fwd = np.zeros((nview, nv, nu))
+        bwd = np.zeros((nview, nz, ny, nx))
+        for v in range(nview):
+            for job in (0, 1):
+                o = fwd[v] if job == 0 else bwd[v]
+                i = np.ascontiguousarray(vol if job == 0 else sino_in[v])
+                r = fn(_p(o), _p(i), nx, ny, nz, s, 2, nu, nv, us, vs,
+                       _p(np.ascontiguousarray(xos[v])), _p(np.ascontiguousarray(xsd[v])), job)
+                assert r == 0
+        out[f"cubic/3d/{name}/fwd"] = fwd
+        out[f"cubic/3d/{name}/bwd"] = bwd
+    img = np.ascontiguousarray(proj2d["img"])
+    ny2, nx2 = img.shape
+    for name in ("fan_a", "fan_edge", "par_a"):
+        nv = int(proj2d[f"{name}/nv"])
+        vs = float(proj2d[f"{name}/vs"])
+        xos, xsd = proj2d[f"{name}/xos"], proj2d[f"{name}/xsd"]
+        sino_in = proj2d[f"{name}/sino_in"]
+        nview = xos.shape[0]
+        fn = getattr(lib, "trtt_FB2D_cubic" if name.startswith("fan") else "trtt_PB2D_cubic")
+        fwd = np.zeros((nview, nv))
+        bwd = np.zeros((nview, ny2, nx2))
+        for v in range(nview):
+            for job in (0, 1):
+                o = fwd[v] if job == 0 else bwd[v]
+                i = np.ascontiguousarray(img if job == 0 else sino_in[v])
+                r = fn(_p(o), _p(i), nx2, ny2, float(proj2d["s"]), 2, nv, vs,
+                       _p(np.ascontiguousarray(xos[v])), _p(np.ascontiguousarray(xsd[v])), job)
+                assert r == 0
+        out[f"cubic/2d/{name}/fwd"] = fwd
+        out[f"cubic/2d/{name}/bwd"] = bwd
     return out
 
 

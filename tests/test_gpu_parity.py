@@ -635,22 +635,24 @@ def test_axial_direct_path_and_edges_vs_oracle(lib, oracle, model):
     P.close()
 
 
-@pytest.mark.parametrize("case", ["quadratic", "perturbed", "short"])
+@pytest.mark.parametrize("case", ["quadratic", "perturbed", "short", "mismatch"])
 def test_table_not_cubic_cdf_uses_general_kernels(lib, oracle, case):
-    """Only a table that holds the cubic B-spline CDF takes the axial
-    kernels: a degree-2 footprint (s_deg = 2, as the reference's 4-D
-    benchmark script builds it), a cubic table with a perturbation of 1e-4,
-    and a 500-entry cubic table (trapezoid error 1.4e-5) run the general
-    kernels, and all match the oracle with their own table."""
+    """Only a table that holds the B-spline CDF of the plan's degree at its
+    nodes takes the axial kernels: a 10^3-entry degree-2 table (trapezoid
+    error 3e-6), a cubic table with a perturbation of 1e-4, a 500-entry
+    cubic table (trapezoid error 1.4e-5), and a cubic table on a degree-2
+    plan run the general kernels, and all match the oracle with their own
+    table."""
     from yoricktrtt_amd.geometry import cone_geometry, footprint_table
     from yoricktrtt_amd.projectors import Projector
 
     _say("table fallback", case)
     nx, ny, nz, nu, nv, V = 22, 20, 18, 30, 34, 4
-    deg = 2 if case == "quadratic" else 3
+    deg = 2 if case in ("quadratic", "mismatch") else 3
     geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, np.linspace(0.1, 4.0, V), 80.0, 130.0,
                         s_deg=deg)
-    _, cum, coord, step = footprint_table(500 if case == "short" else 10000, deg)
+    m = {"short": 500, "quadratic": 1000}.get(case, 10000)
+    _, cum, coord, step = footprint_table(m, 3 if case == "mismatch" else deg)
     if case == "perturbed":
         cum = cum + 1e-4 * np.sin(np.arange(cum.size) * 0.01)
     P = Projector(geo, model="table", precision="f32", table=(cum, coord, step))
@@ -739,7 +741,11 @@ def test_deg2_tables_vs_reference(lib, golden3d, golden2d, precision, m):
         geo = Geometry("cone" if cone else "parallel3d", nx, ny, nz, nu, nv, 1.0, us, vs,
                        np.ascontiguousarray(xos), np.ascontiguousarray(xsd), s_deg=2)
         P = Projector(geo, model="table", precision=precision, table=(cum, coord, step))
-        assert P.path == 0  # a degree-2 footprint runs the general kernels
+        # only FP32 plans whose table holds the quadratic B-spline CDF at its
+        # nodes within 1e-6 (10^4 entries: 3e-8; 10^3 entries: 3e-6) may
+        # take the axial kernels
+        if precision == "f64" or m == 1000:
+            assert P.path == 0
         fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
         bwd = P.backward(_dev(sino_in, dt)).double().cpu().numpy()
         assert rel_err(fwd, rf) < tol and rel_err(bwd, rb.sum(axis=0)) < tol, name
@@ -888,4 +894,110 @@ def test_weighted_residual_refuses_mismatched_arguments(lib):
         P.weighted_residual(s, s.clone(), None, torch.zeros(1, device="cuda:0"))
     with pytest.raises(ValueError):
         P.weighted_residual(s, s.clone(), None, torch.zeros(2, dtype=torch.float64, device="cuda:0"))
+    P.close()
+
+
+def test_cubic_deg2_vs_reference(lib, golden3d, golden2d, precision):
+    """The closed-form cubic symbols called with s_deg = 2 (a truncated
+    cubic footprint: the pixel window of a 1.5-voxel half support) against
+    the reference C outputs of tests/golden/deg2.npz: drop-ins and plans
+    (such plans run the general kernels)."""
+    import os
+
+    from conftest import GOLDEN
+    from yoricktrtt_amd.geometry import Geometry
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("cubic deg2", precision)
+    d = dict(np.load(os.path.join(GOLDEN, "deg2.npz")))
+    tol = TOL[precision]
+    dt = torch.float64 if precision == "f64" else torch.float32
+    vol = np.ascontiguousarray(golden3d["vol"])
+    nz, ny, nx = vol.shape
+    for name in ("cone_a", "cone_edge", "par_a"):
+        nu, nv = (int(v) for v in golden3d[f"{name}/dims"])
+        us, vs = (float(v) for v in golden3d[f"{name}/scl"])
+        xos, xsd = golden3d[f"{name}/xos"], golden3d[f"{name}/xsd"]
+        sino_in = golden3d[f"{name}/sino_in"]
+        cone = name.startswith("cone")
+        fn = lib.trtt_CB3D_cubic if cone else lib.trtt_PB3D_cubic
+        rf, rb = d[f"cubic/3d/{name}/fwd"], d[f"cubic/3d/{name}/bwd"]
+        for v in range(xos.shape[0]):
+            xo, xs = np.ascontiguousarray(xos[v]), np.ascontiguousarray(xsd[v])
+            of = np.zeros((nv, nu))
+            ob = np.zeros((nz, ny, nx))
+            assert fn(_p(of), _p(vol), nx, ny, nz, 1.0, 2, nu, nv, us, vs, _p(xo), _p(xs), 0) == 0
+            assert fn(_p(ob), _p(np.ascontiguousarray(sino_in[v])), nx, ny, nz, 1.0, 2, nu, nv, us,
+                      vs, _p(xo), _p(xs), 1) == 0
+            assert rel_err(of, rf[v]) < tol and rel_err(ob, rb[v]) < tol, (name, v)
+        geo = Geometry("cone" if cone else "parallel3d", nx, ny, nz, nu, nv, 1.0, us, vs,
+                       np.ascontiguousarray(xos), np.ascontiguousarray(xsd), s_deg=2)
+        P = Projector(geo, model="cubic", precision=precision)
+        assert P.path == 0
+        fwd = P.forward(_dev(vol, dt)).double().cpu().numpy()
+        bwd = P.backward(_dev(sino_in, dt)).double().cpu().numpy()
+        assert rel_err(fwd, rf) < tol and rel_err(bwd, rb.sum(axis=0)) < tol, name
+        P.close()
+    img = np.ascontiguousarray(golden2d["img"])
+    ny2, nx2 = img.shape
+    for name in ("fan_a", "fan_edge", "par_a"):
+        nv, vs = int(golden2d[f"{name}/nv"]), float(golden2d[f"{name}/vs"])
+        xos, xsd = golden2d[f"{name}/xos"], golden2d[f"{name}/xsd"]
+        sino_in = golden2d[f"{name}/sino_in"]
+        fan = name.startswith("fan")
+        fn = lib.trtt_FB2D_cubic if fan else lib.trtt_PB2D_cubic
+        rf, rb = d[f"cubic/2d/{name}/fwd"], d[f"cubic/2d/{name}/bwd"]
+        for v in range(xos.shape[0]):
+            xo, xs = np.ascontiguousarray(xos[v]), np.ascontiguousarray(xsd[v])
+            of = np.zeros(nv)
+            ob = np.zeros((ny2, nx2))
+            assert fn(_p(of), _p(img), nx2, ny2, 1.0, 2, nv, vs, _p(xo), _p(xs), 0) == 0
+            assert fn(_p(ob), _p(np.ascontiguousarray(sino_in[v])), nx2, ny2, 1.0, 2, nv, vs,
+                      _p(xo), _p(xs), 1) == 0
+            assert rel_err(of, rf[v]) < tol and rel_err(ob, rb[v]) < tol, (name, v)
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel3d"])
+def test_axial_deg2_table_vs_oracle(lib, oracle, kind):
+    """A degree-2 table of 10^4 entries (within 3e-8 of the quadratic
+    B-spline CDF at its nodes) on circular orbits runs the axial kernels with
+    the quadratic CDF evaluated at the table nodes: every forward tile and
+    both backward tile kinds against the FP64 oracle of the same table (the
+    oracle is pinned to the reference C by the degree-2 golden vectors)."""
+    from yoricktrtt_amd import autotune as AT
+    from yoricktrtt_amd.geometry import cone_geometry, footprint_table, parallel_geometry_3d
+    from yoricktrtt_amd.projectors import Projector
+
+    _say("axial deg2 table", kind)
+    nx, ny, nz, nu, nv, V = 22, 19, 70, 90, 40, 6
+    rng = np.random.default_rng(88)
+    th = rng.uniform(0, 2 * math.pi, size=V)
+    if kind == "cone":
+        geo = cone_geometry(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, th, 120.0, 190.0, v_off=0.7,
+                            s_deg=2)
+    else:
+        geo = parallel_geometry_3d(nx, ny, nz, 1.0, nu, nv, 0.8, 0.9, th, s_deg=2)
+    _, cum, coord, step = footprint_table(10000, 2)
+    P = Projector(geo, model="table", precision="f32", table=(cum, coord, step))
+    assert P.path > 0
+    vol = rng.uniform(size=(nz, ny, nx))
+    sino = rng.uniform(size=(V, nv, nu))
+    tab = (cum, coord, step)
+    us, vs = geo.u_scl, geo.v_scl
+    rf = np.stack([oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, vs,
+                               geo.xos[v], geo.xsd[v], kind == "cone", 0, tab, deg=2)
+                   for v in range(V)])
+    rb = np.zeros((nz, ny, nx))
+    for v in range(V):
+        oracle.sb3d(rb, sino[v], nx, ny, nz, 1.0, nu, nv, us, vs, geo.xos[v], geo.xsd[v],
+                    kind == "cone", 1, tab, deg=2)
+    for tile in sorted(AT.FWD_TILES):
+        AT.set_tuning(P, fwd_tile=tile, fwd_window_bytes=16384)
+        fwd = P.forward(_dev(vol, torch.float32)).double().cpu().numpy()
+        assert rel_err(fwd, rf) < 1e-5, tile
+    for tile in (0, 3):
+        AT.set_tuning(P, bwd_tile=tile, bwd_window_bytes=32768)
+        bwd = P.backward(_dev(sino, torch.float32)).double().cpu().numpy()
+        assert rel_err(bwd, rb) < 1e-5, tile
+    assert P.error_flag() == 0
     P.close()

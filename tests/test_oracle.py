@@ -294,3 +294,41 @@ def test_oracle_deg2_tables_vs_reference(oracle, golden3d, golden2d, m):
             rb = oracle.sb2d(np.zeros((ny2, nx2)), golden2d[f"{name}/sino_in"][v], nx2, ny2, 1.0,
                              nv, vs, xos[v], xsd[v], fan, 1, tab, deg=2)
             assert rel_err(rb, d[f"m{m}/2d/{name}/bwd"][v]) < 1e-14, (name, v)
+
+
+def test_oracle_cubic_deg2_vs_reference(oracle, golden3d, golden2d):
+    """The closed-form cubic symbols called with s_deg = 2 (the cubic CDF
+    on the pixel window of a 1.5-voxel half support) against the reference C
+    outputs of deg2.npz."""
+    d = dict(np.load(os.path.join(GOLDEN, "deg2.npz")))
+    vol = golden3d["vol"]
+    nz, ny, nx = vol.shape
+    for name in ("cone_a", "cone_edge", "par_a"):
+        nu, nv = (int(v) for v in golden3d[f"{name}/dims"])
+        us, vs = (float(v) for v in golden3d[f"{name}/scl"])
+        xos, xsd = golden3d[f"{name}/xos"], golden3d[f"{name}/xsd"]
+        cone = name.startswith("cone")
+        for v in range(xos.shape[0]):
+            rf = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, vs, xos[v],
+                             xsd[v], cone, 0, None, deg=2)
+            assert rel_err(rf, d[f"cubic/3d/{name}/fwd"][v]) < 1e-13, (name, v)
+            rb = oracle.sb3d(np.zeros((nz, ny, nx)), golden3d[f"{name}/sino_in"][v], nx, ny, nz,
+                             1.0, nu, nv, us, vs, xos[v], xsd[v], cone, 1, None, deg=2)
+            assert rel_err(rb, d[f"cubic/3d/{name}/bwd"][v]) < 1e-13, (name, v)
+            # not the s_deg = 3 result (the truncation matters)
+            r3 = oracle.sb3d(np.zeros((nv, nu)), vol, nx, ny, nz, 1.0, nu, nv, us, vs, xos[v],
+                             xsd[v], cone, 0, None, deg=3)
+            assert rel_err(r3, rf) > 1e-4
+    img = golden2d["img"]
+    ny2, nx2 = img.shape
+    for name in ("fan_a", "fan_edge", "par_a"):
+        nv, vs = int(golden2d[f"{name}/nv"]), float(golden2d[f"{name}/vs"])
+        xos, xsd = golden2d[f"{name}/xos"], golden2d[f"{name}/xsd"]
+        fan = name.startswith("fan")
+        for v in range(xos.shape[0]):
+            rf = oracle.sb2d(np.zeros(nv), img, nx2, ny2, 1.0, nv, vs, xos[v], xsd[v], fan, 0,
+                             None, deg=2)
+            assert rel_err(rf, d[f"cubic/2d/{name}/fwd"][v]) < 1e-13, (name, v)
+            rb = oracle.sb2d(np.zeros((ny2, nx2)), golden2d[f"{name}/sino_in"][v], nx2, ny2, 1.0,
+                             nv, vs, xos[v], xsd[v], fan, 1, None, deg=2)
+            assert rel_err(rb, d[f"cubic/2d/{name}/bwd"][v]) < 1e-13, (name, v)

@@ -56,6 +56,23 @@ __device__ __forceinline__ T cdf3c(T x) {
   return copysign(fma(r2, r2, fma(s2, -s2, T(0.5))), x);
 }
 
+// F(x) - 1/2 for the quadratic B-spline CDF, by the same construction: on
+// [0, 3/2], G(t) = 1/2 - (3/2-t)^3/6 + max(1/2-t, 0)^3/2 (the pieces
+// 3t/4 - t^3/3 on [0, 1/2] and 1/2 - (3/2-t)^3/6 on [1/2, 3/2] differ by
+// exactly (1/2-t)^3/2), with s = max(3/2-t, 0)/6^(1/3) and
+// r = max(1/2-t, 0)/2^(1/3), both below 1: G = 1/2 - s^2 s + r^2 r.
+constexpr double kQ6 = 0.5503212081491045;   // 6^(-1/3)
+constexpr double kQ2 = 0.7937005259840998;   // 2^(-1/3)
+
+template <typename T>
+__device__ __forceinline__ T cdf2c(T x) {
+  const T t = fabs(x);
+  const T s = fmin(fmax(fma(t, T(-kQ6), T(1.5 * kQ6)), T(0)), T(1));
+  const T r = fmin(fmax(fma(t, T(-kQ2), T(0.5 * kQ2)), T(0)), T(1));
+  const T s2 = s * s, r2 = r * r;
+  return copysign(fma(r2, r, fma(s2, -s, T(0.5))), x);
+}
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -110,12 +127,13 @@ struct Col {
 // for this path only when it covers the footprint's support [-2, 2], so
 // past either end the CDF at the node is the constant the clamped entry
 // holds (-1/2 or +1/2 here) and no clamp is needed.
-template <typename T>
+template <int TAB, typename T>
 __device__ __forceinline__ T table_cdf(const DevPlan &P, double x) {
-  return cdf3c<T>(fmaf(float(floor(x)), float(P.step), float(P.coord0)));
+  const float node = fmaf(float(floor(x)), float(P.step), float(P.coord0));
+  return TAB == 2 ? cdf2c<T>(node) : cdf3c<T>(node);
 }
 
-template <bool CONE, bool TAB, typename T>
+template <bool CONE, int TAB, typename T>
 __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int i, int j, Col<T> &c) {
 #pragma clang fp contract(off)
   const double di = double(i), dj = double(j);
@@ -200,7 +218,7 @@ __device__ __forceinline__ bool on_detector(const Vox<T> &v) {
   return v.ur > max(v.ul, 0);
 }
 
-template <bool TAB = false, typename T>
+template <int TAB = 0, typename T>
 __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T> &c, int k,
                                       Vox<T> &v) {
   v.uc = c.uc0 + c.duc * double(k);
@@ -237,13 +255,13 @@ __device__ __forceinline__ void voxel(const DevPlan &P, double a12, const Col<T>
 // weight is 1/2 - F because the CDF is exactly 1/2 from the footprint's right
 // end on.  The window starts (unclamped) at least 2h left of the centre,
 // where the CDF is exactly -1/2.  Entries past the window are zero.
-template <int NP, bool TAB = false, typename T>
+template <int NP, int TAB = 0, typename T>
 __device__ __forceinline__ void uweights(const DevPlan &P, const Vox<T> &v, T (&w)[NP]) {
   const T x0 = T(double(v.ul) - v.uc) * v.su;
   T prev = T(-0.5);
 #pragma unroll
   for (int l = 0; l + 1 < NP; ++l) {
-    const T nx = TAB ? table_cdf<T>(P, fma(double(l + 1), v.sdu, v.xu0))
+    const T nx = TAB ? table_cdf<TAB, T>(P, fma(double(l + 1), v.sdu, v.xu0))
                      : cdf3c<T>(x0 + T(l + 1) * v.su);
     w[l] = nx - prev;
     prev = nx;
@@ -279,9 +297,9 @@ __device__ __forceinline__ float from_next_lane(float v) {
 // v weights of the column: lane m evaluates the CDF at the m-th v pixel
 // bound, one shifted difference gives the weight of pixel m in lane m (zero
 // from the window's end on), and readlane broadcasts the W weights.
-template <int W, bool TAB = false, typename T>
+template <int W, int TAB = 0, typename T>
 __device__ __forceinline__ void vweights(const DevPlan &P, const Col<T> &c, int lane, T (&cv)[W]) {
-  const T f = TAB ? table_cdf<T>(P, fma(double(lane), c.sdv, c.xv0))
+  const T f = TAB ? table_cdf<TAB, T>(P, fma(double(lane), c.sdv, c.xv0))
                   : cdf3c<T>((T(lane) - c.vcf) * c.sv);
   // the shift runs on all lanes (a lane reading a disabled lane gets 0)
   const T fn = from_next_lane(f);
@@ -457,7 +475,7 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-template <int NP, bool TAB, typename T>
+template <int NP, int TAB, typename T>
 __device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v, T *pp, T val,
                                               int passes) {
   T w[NP];
@@ -491,7 +509,7 @@ __device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v,
 // Backward gather of one voxel whose window has at most NP pixels against
 // the column's Q row (qq: Q at the window's first pixel; Q is zero past the
 // column span).
-template <int NP, bool TAB, typename T>
+template <int NP, int TAB, typename T>
 __device__ __forceinline__ T gather_steps(const DevPlan &P, const Vox<T> &v, const T *qq) {
   T w[NP];
   uweights<NP, TAB>(P, v, w);
@@ -502,7 +520,7 @@ __device__ __forceinline__ T gather_steps(const DevPlan &P, const Vox<T> &v, con
 }
 
 // Dispatch on the column's wave-uniform pixel bound np (1..W).
-template <int W, bool TAB, typename T>
+template <int W, int TAB, typename T>
 __device__ __forceinline__ void profile_n(const DevPlan &P, int np, const Vox<T> &v, T *pp, T val,
                                           int pl) {
   switch (np) {
@@ -517,7 +535,7 @@ __device__ __forceinline__ void profile_n(const DevPlan &P, int np, const Vox<T>
   }
 }
 
-template <int W, bool TAB, typename T>
+template <int W, int TAB, typename T>
 __device__ __forceinline__ T gather_n(const DevPlan &P, int np, const Vox<T> &v, const T *qq) {
   switch (np) {
 #define TRTT_GS(NN) \
@@ -531,7 +549,7 @@ __device__ __forceinline__ T gather_n(const DevPlan &P, int np, const Vox<T> &v,
 }
 
 // Slow path for one voxel: arbitrary window sizes, direct global memory.
-template <bool FWD, bool TAB, typename T>
+template <bool FWD, int TAB, typename T>
 __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T val, T *out,
                            const T *in, T &acc) {
   const int ul = max(v.ul, 0);  // the part of the window on the detector
@@ -541,11 +559,11 @@ __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T
   const T xv0 = -c.vcf * c.sv;
   // tabulated footprint: u coordinates from the window start ul (>= v.ul)
   auto fu = [&](int l) {
-    return TAB ? table_cdf<T>(P, fma(double(ul - v.ul + l), v.sdu, v.xu0))
+    return TAB ? table_cdf<TAB, T>(P, fma(double(ul - v.ul + l), v.sdu, v.xu0))
                : cdf3c<T>(xu0 + T(l) * v.su);
   };
   auto fv = [&](int m) {
-    return TAB ? table_cdf<T>(P, fma(double(m), c.sdv, c.xv0)) : cdf3c<T>(xv0 + T(m) * c.sv);
+    return TAB ? table_cdf<TAB, T>(P, fma(double(m), c.sdv, c.xv0)) : cdf3c<T>(xv0 + T(m) * c.sv);
   };
   T fv0 = fv(0);
   T sum = T(0);
@@ -635,7 +653,7 @@ struct ColLane {
 };
 
 // Column `col` of the tile on view g; `active` false leaves the lane empty.
-template <bool CONE, bool TAB, int CX, int CY, typename T>
+template <bool CONE, int TAB, int CX, int CY, typename T>
 __device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &g, const TileMap &tm,
                                              int col, bool active, int kf, int kl, ColLane<T> &cl) {
   cl.i = tm.bx * CX + col % CX;
@@ -655,7 +673,7 @@ __device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &
   }
 }
 
-template <bool CONE, bool TAB, int CX, int CY, int NCOL, typename T>
+template <bool CONE, int TAB, int CX, int CY, int NCOL, typename T>
 __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &g, const TileMap &tm,
                                              int wave, int lane, int kf, int kl, ColLane<T> &cl) {
   column_entry<CONE, TAB, CX, CY>(P, g, tm, wave * NCOL + lane, lane < NCOL, kf, kl, cl);
@@ -663,7 +681,7 @@ __device__ __forceinline__ void column_table(const DevPlan &P, const ViewConst &
 
 __device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-template <bool TAB = false, typename T>
+template <int TAB = 0, typename T>
 __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> &c, int &lo, int &hi,
                                           int &np, int &i, int &j) {
   if (TAB) {
@@ -746,7 +764,7 @@ __device__ __forceinline__ void block_window(const ColLane<T> &cl, bool live, in
 #define TRTT_BWD_OCC
 #endif
 
-template <bool CONE, int W, int CX, int CY, bool TAB, typename T>
+template <bool CONE, int W, int CX, int CY, int TAB, typename T>
 __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__restrict__ vol,
                                           T *__restrict__ sino, int views_per_block, int wcap,
                                           int pmax, int view_base, int view_end,
@@ -1048,7 +1066,7 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
 // VPL voxels per lane: the tile spans 64 VPL slices and lane L holds the
 // voxels k = kf + 64 q + L, q < VPL, so that the per-column set-up (column
 // broadcast, v weights, Q row) serves 64 VPL voxels.
-template <bool CONE, int W, int CX, int CY, bool TWO, int VPL, bool TAB, typename T>
+template <bool CONE, int W, int CX, int CY, bool TWO, int VPL, int TAB, typename T>
 __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T *__restrict__ sino,
                                                 T *__restrict__ vol,
                                                 const int *__restrict__ seg_begin, int atomic_out,
@@ -1299,12 +1317,16 @@ template <bool CONE, int W, int CX, int CY, typename T>
 void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *volk, T *out,
                 int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain,
                 int flush) {
-  if (P.table)
-    hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, true, T>), grid, dim3(64), lds_bytes, st, P,
-                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush);
+#define TRTT_FWD(TB)                                                                           \
+  hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, TB, T>), grid, dim3(64), lds_bytes, st, P, volk, \
+                     out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush)
+  if (!P.table)
+    TRTT_FWD(0);
+  else if (P.table_deg == 2)
+    TRTT_FWD(2);
   else
-    hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, false, T>), grid, dim3(64), lds_bytes, st, P,
-                       volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush);
+    TRTT_FWD(3);
+#undef TRTT_FWD
 }
 
 template <bool CONE, int W, int CX, int CY, int VPL, typename T>
@@ -1313,10 +1335,12 @@ void launch_bwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, cons
 #define TRTT_BWD(TW, TB)                                                                        \
   hipLaunchKernelGGL((axial::bwd<CONE, W, CX, CY, TW, VPL, TB, T>), grid, dim3(axial::kThreads), \
                      lds_bytes, st, P, in, out, seg_begin, atomic_out, lds_cap, pmax)
-  if (P.table) {
-    if (two) TRTT_BWD(true, true); else TRTT_BWD(false, true);
+  if (!P.table) {
+    if (two) TRTT_BWD(true, 0); else TRTT_BWD(false, 0);
+  } else if (P.table_deg == 2) {
+    if (two) TRTT_BWD(true, 2); else TRTT_BWD(false, 2);
   } else {
-    if (two) TRTT_BWD(true, false); else TRTT_BWD(false, false);
+    if (two) TRTT_BWD(true, 3); else TRTT_BWD(false, 3);
   }
 #undef TRTT_BWD
 }

@@ -186,27 +186,43 @@ void view_consts_lf(ViewConst &g, const lf2d_parameters_t *p, double beta, doubl
 // rotate about the object z axis (o3 = den_z = num_v_z = 0), with the whole
 // volume in front of the source.  Returns the footprint width bound W
 // (9 or 13 pixels) and the column profile length, or W = 0.
-// A footprint table that holds the cubic B-spline CDF at its nodes,
-// cum[n] = F(coord0 + n step) - F(coord0) to within 1e-6 (the trapezoid
-// tables of Ytrtt_tomobj.i:140-196 are within 4e-8 for 10^4 entries), lets
-// the axial kernels evaluate the table entry of an index in closed form.
-bool table_is_cubic_cdf(const double *cum, long m, double step, double coord0) {
-  if (!cum || m < 2 || !(step > 0.0)) return false;
-  // the table covers the support [-2, 2] (the axial kernels skip the clamp)
-  if (!(coord0 <= -2.0) || !(coord0 + double(m - 1) * step >= 2.0)) return false;
-  const double f0 = trtt::host_cdf3(coord0);
+// CDF of the quadratic B-spline (support [-3/2, 3/2]), FP64, piecewise.
+double host_cdf2(double x) {
+  if (x <= -1.5) return 0.0;
+  if (x <= -0.5) return (x + 1.5) * (x + 1.5) * (x + 1.5) / 6.0;
+  if (x <= 0.5) return 0.5 + 0.75 * x - x * x * x / 3.0;
+  if (x <= 1.5) return 1.0 - (1.5 - x) * (1.5 - x) * (1.5 - x) / 6.0;
+  return 1.0;
+}
+
+// A footprint table of degree deg (2 or 3) that holds the B-spline CDF at
+// its nodes, cum[n] = F(coord0 + n step) - F(coord0) to within 1e-6 (the
+// trapezoid tables of Ytrtt_tomobj.i:140-196 are within 4e-8 for 10^4
+// entries), lets the axial kernels evaluate the table entry of an index in
+// closed form.  Returns deg when it does, else 0.
+int table_cdf_degree(const double *cum, long m, double step, double coord0, int deg) {
+  if (!cum || m < 2 || !(step > 0.0) || (deg != 2 && deg != 3)) return 0;
+  // the table covers the support [-hs, hs] (the axial kernels skip the clamp)
+  const double hs = 0.5 * (deg + 1);
+  if (!(coord0 <= -hs) || !(coord0 + double(m - 1) * step >= hs)) return 0;
+  auto F = [deg](double x) { return deg == 2 ? host_cdf2(x) : trtt::host_cdf3(x); };
+  const double f0 = F(coord0);
   for (long n = 0; n < m; ++n)
-    if (!(std::fabs(cum[n] - (trtt::host_cdf3(coord0 + double(n) * step) - f0)) <= 1e-6))
-      return false;
-  return true;
+    if (!(std::fabs(cum[n] - (F(coord0 + double(n) * step) - f0)) <= 1e-6)) return 0;
+  return deg;
 }
 
 void axial_analysis(trtt_hip_plan *p) {
   p->axial_w = 0;
   p->axial_pmax = 0;
   p->axial_pmax2 = 0;
-  // tabulated footprints: only a cubic CDF table (closed form at the nodes)
-  if ((p->table && !(p->table_cdf3 && p->s_deg == 3)) || p->precision != TRTT_HIP_F32) return;
+  // the kernels take the CDF as exactly -1/2 and 1/2 at a voxel window's
+  // outer bounds: the closed-form cubic with s_deg 3 (with another degree it
+  // is truncated), or a table that holds the CDF of the B-spline of degree
+  // s_deg (2 or 3) at its nodes (evaluated in closed form there)
+  const bool cubic_ok = !p->table && p->s_deg == 3;
+  const bool table_ok = p->table && p->table_deg == p->s_deg;
+  if (!(cubic_ok || table_ok) || p->precision != TRTT_HIP_F32) return;
   if (p->kind != trtt::CB3D && p->kind != trtt::PB3D) return;
   // the kernels broadcast detector and column indices packed in 16 bits
   if (p->nu > 65535 || p->nv > 65535 || p->nx > 65535 || p->ny > 65535) return;
@@ -604,7 +620,8 @@ trtt_hip_plan *make_plan(int kind, int precision, long nx, long ny, long nz, lon
     release(p);
     return nullptr;
   }
-  p->table_cdf3 = p->table && table_is_cubic_cdf(cum, m, step, coord0);
+  p->table_deg = p->table ? table_cdf_degree(cum, m, step, coord0, s_deg) : 0;
+  p->dev.table_deg = p->table_deg;
   axial_analysis(p);
   if (p->axial_w > 0 && !alloc_axial_scratch(p)) {
     release(p);

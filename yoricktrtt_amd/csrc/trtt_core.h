@@ -57,6 +57,7 @@ struct DevPlan {
   long vox_per_phase;      // nx*ny*nz
   double u_scl, v_scl;
   double hu_init, hv_init, rho_init, hs0;
+  int table_deg;           // axial kernels: degree of the B-spline CDF the table holds
   // Long & Fessler scalars
   double lf_vs, lf_ps, lf_R, lf_scale, lf_size;
   int lf_line;             // 1: point-sampled (line-integral) trapezoid model
@@ -121,7 +122,8 @@ struct trtt_hip_plan {
   int precision = TRTT_HIP_F32;
   int device = 0;
   bool table = false;
-  bool table_cdf3 = false;  // the table is the cubic B-spline CDF (axial path allowed)
+  int table_deg = 0;  // the table holds the CDF of the B-spline of this degree (2 or 3) at
+                      // its nodes (axial path allowed), 0 otherwise
   bool quirk_last_bin = false;
   long nx = 0, ny = 0, nz = 0, nt = 1, nu = 1, nv = 1, nviews = 0;
   double s_scl = 1.0, u_scl = 1.0, v_scl = 1.0;
