@@ -114,6 +114,8 @@ struct Col {
   // tabulated footprint: FP64 table coordinate of the window start and
   // its step per pixel along v; hu / du and 1 / (a4^2 + vca8^2) in FP64
   double xv0, sdv, hu0d, inv_ad;
+  // closed-form footprint: 24^(-1/4) / hu0 (the symmetric evaluation)
+  T ca;
 };
 
 // Tabulated footprint (trtt_PB3D / trtt_CB3D with a table of the cubic
@@ -155,6 +157,7 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.sv = T(1.0 / hv);
     c.rho = T(P.rho_init * magn * magn * dv);
     c.hu0 = T(P.hu_init * magn);
+    c.ca = T(kC24 / (P.hu_init * magn));
     c.inv_a = T(1.0 / a);
     clampw(c.vc, P.hs0 * hv, P.nv, c.vl, c.vr);
     c.vcf = T(c.vc - double(c.vl));
@@ -174,6 +177,7 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.sv = T(g.pb_sv);
     c.rho = T(g.pb_rho0);
     c.hu0 = T(g.pb_hu);
+    c.ca = T(kC24 / g.pb_hu);
     c.inv_a = T(0);
     clampw(c.vc, g.pb_hsv, P.nv, c.vl, c.vr);
     c.vcf = T(c.vc - double(c.vl));
@@ -284,6 +288,98 @@ __device__ __forceinline__ bool fits(const Vox<T> &v, int np, int nu) {
 // the window bounds; voxels that still exceed it take the slow path.
 __device__ __forceinline__ int pixel_bound(float hs_a, float hs_b) {
   return int(ceilf(2.0f * fmaxf(hs_a, hs_b) + 1.0e-3f)) + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric evaluation of the closed-form cubic footprint (no table).
+// The pixel bounds are counted outwards from the pixel fl = floor(uc) that
+// holds the centre: with f = uc - fl, bound j on the left lies at
+// x = -(f + j) su and bound j on the right at x = (1 - f + j) su
+// (j = 0, 1, ...; x in units of hu).  On the left half the CDF is
+// F(x) = s^4 - r^4 with s = max(2 + x, 0)/24^(1/4), r = max(1 + x, 0)/6^(1/4)
+// (the truncated powers (x+2)^4/24 - 4 (x+1)^4/24 of the cubic B-spline);
+// on the right half 1 - F(x) has the same form in -x.  With
+// A = su/24^(1/4) and A' = su/6^(1/4) the bases of bound j are
+// (2/24^(1/4) - A f) - j A and (1/6^(1/4) - A' f) - j A', both below 1, so
+// each is one FMA whose [0, 1] output clamp is the max(., 0): six
+// instructions per bound, with neither |x| nor a sign transfer.  Past the
+// support (j >= hs - f on the left, hs = 2 hu the half support in pixels)
+// both forms are exactly 0, so J = floor(hs) bounds per side cover the
+// window fl - J - 1 .. fl + J + 1 of 2J + 3 pixels:
+//   w(fl - 1 - j) = FL_j - FL_(j+1),  w(fl) = 1 - FL_0 - FR_0,
+//   w(fl + 1 + j) = FR_j - FR_(j+1),  FL_(J+1) = FR_(J+1) = 0
+// with FL_j the CDF at left bound j and FR_j the tail 1 - F at right
+// bound j; the weights telescope to 1 (up to rounding).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double frsq(double x) { return 1.0 / sqrt(x); }
+
+// Centre of voxel kd of a column in FP64: every use (column hull, voxels)
+// takes this one expression, so the floors agree.
+template <typename T>
+__device__ __forceinline__ double col_uc(const Col<T> &c, double kd) {
+  return fma(c.duc, kd, c.uc0);
+}
+
+template <typename T>
+struct Sym {
+  T A, A2;            // su / 24^(1/4), su / 6^(1/4)
+  T bl, rl, br, rr;   // bases of bound 0: s and r on the left, s and r on the right
+  T rho;              // rho0 du
+  int p;              // centre pixel fl relative to the column profile start
+};
+
+// a12s = a12 u_scl; lo: first pixel of the column profile.  The distortion
+// factor takes the centre in single precision (its relative error reaches
+// du only through (uc - a12)^2 / a, far below single precision); the
+// sub-pixel offset f comes from the FP64 centre.
+template <typename T>
+__device__ __forceinline__ void sym_voxel(const DevPlan &P, T a12s, const Col<T> &c, double kd,
+                                          int lo, Sym<T> &s) {
+  const double uc = col_uc(c, kd);
+  const double fl = floor(uc);
+  const T f = T(uc - fl);
+  s.p = int(fl) - lo;
+  const T uca = fma(T(uc), T(P.u_scl), -a12s);
+  const T dsq = fma(uca * c.inv_a, uca, T(1));
+  const T rs = frsq(dsq);  // 1 / du
+  s.A = c.ca * rs;
+  s.A2 = s.A * T(kC6 / kC24);
+  s.rho = c.rho * (dsq * rs);
+  const T g = T(1) - f;
+  s.bl = fma(-s.A, f, T(2.0 * kC24));
+  s.rl = fma(-s.A2, f, T(kC6));
+  s.br = fma(-s.A, g, T(2.0 * kC24));
+  s.rr = fma(-s.A2, g, T(kC6));
+}
+
+template <typename T>
+__device__ __forceinline__ T sym_cdf(T bs, T br) {
+  const T s = fmin(fmax(bs, T(0)), T(1)), r = fmin(fmax(br, T(0)), T(1));
+  const T s2 = s * s, r2 = r * r;
+  return fma(s2, s2, -(r2 * r2));
+}
+
+// Weights of the pixels fl - J - 1 + t, t < 2J + 3.
+template <int J, typename T>
+__device__ __forceinline__ void sym_weights(const Sym<T> &s, T (&w)[2 * J + 3]) {
+  T L[J + 1], R[J + 1];
+#pragma unroll
+  for (int j = 0; j <= J; ++j) {
+    const T bl = j == 0 ? s.bl : fma(T(-j), s.A, s.bl);
+    const T rl = j == 0 ? s.rl : fma(T(-j), s.A2, s.rl);
+    const T br = j == 0 ? s.br : fma(T(-j), s.A, s.br);
+    const T rr = j == 0 ? s.rr : fma(T(-j), s.A2, s.rr);
+    L[j] = sym_cdf(bl, rl);
+    R[j] = sym_cdf(br, rr);
+  }
+  w[0] = L[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) w[J - j] = L[j] - L[j + 1];
+  w[J + 1] = (T(1) - L[0]) - R[0];
+#pragma unroll
+  for (int j = 0; j < J; ++j) w[J + 2 + j] = R[j] - R[j + 1];
+  w[2 * J + 2] = R[J];
 }
 
 // Lane L receives v from lane L + 1 (DPP wave shift; the last lane's value
@@ -475,11 +571,8 @@ __device__ __forceinline__ T q_n(int n, const T *src, int RW, const T (&cv)[W]) 
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-template <int NP, int TAB, typename T>
-__device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v, T *pp, T val,
-                                              int passes) {
-  T w[NP];
-  uweights<NP, TAB>(P, v, w);
+template <int NP, typename T>
+__device__ __forceinline__ void accumulate_steps(const T (&w)[NP], T *pp, T val, int passes) {
   // plain steps: one FMA per pixel into the profile entry (the product is
   // not formed apart, so it is not shared with the atomic branch)
   if (passes == 1) {
@@ -506,6 +599,22 @@ __device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v,
   }
 }
 
+template <int NP, int TAB, typename T>
+__device__ __forceinline__ void profile_steps(const DevPlan &P, const Vox<T> &v, T *pp, T val,
+                                              int passes) {
+  T w[NP];
+  uweights<NP, TAB>(P, v, w);
+  accumulate_steps<NP>(w, pp, val, passes);
+}
+
+template <int NP, typename T>
+__device__ __forceinline__ T dot_steps(const T (&w)[NP], const T *qq) {
+  T sum = T(0);
+#pragma unroll
+  for (int l = 0; l < NP; ++l) sum += w[l] * qq[l];
+  return sum;
+}
+
 // Backward gather of one voxel whose window has at most NP pixels against
 // the column's Q row (qq: Q at the window's first pixel; Q is zero past the
 // column span).
@@ -513,10 +622,44 @@ template <int NP, int TAB, typename T>
 __device__ __forceinline__ T gather_steps(const DevPlan &P, const Vox<T> &v, const T *qq) {
   T w[NP];
   uweights<NP, TAB>(P, v, w);
-  T sum = T(0);
-#pragma unroll
-  for (int l = 0; l < NP; ++l) sum += w[l] * qq[l];
-  return sum;
+  return dot_steps<NP>(w, qq);
+}
+
+// Symmetric footprints with a column-uniform J (2J + 3 <= W): profile steps
+// (pp: the profile at pixel fl - J - 1) and gathers (qq: Q at that pixel).
+template <int W, typename T>
+__device__ __forceinline__ void sym_profile_n(int J, const Sym<T> &s, T *pp, T val, int passes) {
+  switch (J) {
+#define TRTT_SP(JJ)                                 \
+  case JJ:                                          \
+    if constexpr (2 * JJ + 3 <= W) {                \
+      T w[2 * JJ + 3];                              \
+      sym_weights<JJ>(s, w);                        \
+      accumulate_steps<2 * JJ + 3>(w, pp, val, passes); \
+    }                                               \
+    break;
+    TRTT_SP(0) TRTT_SP(1) TRTT_SP(2) TRTT_SP(3) TRTT_SP(4) TRTT_SP(5)
+#undef TRTT_SP
+    default: break;
+  }
+}
+
+template <int W, typename T>
+__device__ __forceinline__ T sym_gather_n(int J, const Sym<T> &s, const T *qq) {
+  switch (J) {
+#define TRTT_SG(JJ)                    \
+  case JJ:                             \
+    if constexpr (2 * JJ + 3 <= W) {   \
+      T w[2 * JJ + 3];                 \
+      sym_weights<JJ>(s, w);           \
+      return dot_steps<2 * JJ + 3>(w, qq); \
+    }                                  \
+    break;
+    TRTT_SG(0) TRTT_SG(1) TRTT_SG(2) TRTT_SG(3) TRTT_SG(4) TRTT_SG(5)
+#undef TRTT_SG
+    default: break;
+  }
+  return T(0);
 }
 
 // Dispatch on the column's wave-uniform pixel bound np (1..W).
@@ -644,13 +787,39 @@ __device__ __forceinline__ void column_hull(const DevPlan &P, const ViewConst &g
 template <typename T>
 struct ColLane {
   Col<T> c;
-  int lo, hi;   // u hull of the column over [kf, kl]
-  int np;       // pixel bound of its voxel windows
+  int lo, hi;   // u hull of the column over [kf, kl] on the detector
+  int np;       // pixel bound of its voxel windows (symmetric windows: J)
   int i, j;     // voxel column
   int base;     // (j nx + i) nz: the column's offset in the k-fastest volume copy
+  // symmetric windows: first pixel of the column profile (not clamped to
+  // the detector) and its length | J << 16
+  int ulo, span_j;
   bool live;    // inside the volume, wk != 0, non-empty windows
   bool bad;     // wk == 0 (voxel in the source plane)
 };
+
+// Column hull of the symmetric windows over [kf, kl]: J = floor(hs_max),
+// hs_max the larger half support of the two ends (hs is convex in k), and
+// the profile spans fl_min - J - 1 .. fl_max + J + 1 with fl the floors of
+// the centres at the two ends (uc is affine in k and col_uc monotone in k,
+// so every centre of the column lies between them).
+template <typename T>
+__device__ __forceinline__ void column_hull_sym(const DevPlan &P, const ViewConst &g,
+                                                const Col<T> &c, int kf, int kl, ColLane<T> &cl) {
+  Vox<T> a, b;
+  voxel(P, g.a12, c, kf, a);
+  voxel(P, g.a12, c, kl, b);
+  const float hs = fmaxf(a.hs, b.hs);
+  const int J = hs < 64.0f ? int(floorf(hs)) : 64;
+  const double fa = floor(col_uc(c, double(kf))), fb = floor(col_uc(c, double(kl)));
+  const double lo = fmin(fa, fb) - double(J + 1), hi = fmax(fa, fb) + double(J + 2);
+  // far-off columns saturate: their span fails every profile bound
+  cl.ulo = int(fmin(fmax(lo, -1.0e9), 1.0e9));
+  cl.span_j = int(fmin(fmax(hi - lo, 0.0), 65535.0)) | (min(J, 255) << 16);
+  cl.lo = int(fmin(fmax(lo, 0.0), double(P.nu)));
+  cl.hi = int(fmin(fmax(hi, 0.0), double(P.nu)));
+  cl.np = J;
+}
 
 // Column `col` of the tile on view g; `active` false leaves the lane empty.
 template <bool CONE, int TAB, int CX, int CY, typename T>
@@ -663,11 +832,16 @@ __device__ __forceinline__ void column_entry(const DevPlan &P, const ViewConst &
   cl.bad = false;
   cl.lo = cl.hi = 0;
   cl.np = 0;
+  cl.ulo = 0;
+  cl.span_j = 0;
   if (active && cl.i < P.nx && cl.j < P.ny) {
     column<CONE, TAB>(P, g, cl.i, cl.j, cl.c);
     cl.bad = !cl.c.ok;
     if (cl.c.ok && cl.c.vl < cl.c.vr) {
-      column_hull(P, g, cl.c, kf, kl, cl.lo, cl.hi, cl.np);
+      if (TAB == 0)
+        column_hull_sym(P, g, cl.c, kf, kl, cl);
+      else
+        column_hull(P, g, cl.c, kf, kl, cl.lo, cl.hi, cl.np);
       cl.live = cl.lo < cl.hi;
     }
   }
@@ -709,6 +883,27 @@ __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> 
   const int ij = bcast_i(src.i | (src.j << 16), cc);
   i = ij & 0xffff;
   j = int(unsigned(ij) >> 16);
+}
+
+// Column cc for the symmetric windows: lo, span and J of its profile.
+template <typename T>
+__device__ __forceinline__ void bcast_sym(const ColLane<T> &src, int cc, Col<T> &c, int &lo,
+                                          int &span, int &J) {
+  c.vcf = bcast(src.c.vcf, cc);
+  c.uc0 = bcast(src.c.uc0, cc);
+  c.duc = bcast(src.c.duc, cc);
+  c.rho = bcast(src.c.rho, cc);
+  c.ca = bcast(src.c.ca, cc);
+  c.inv_a = bcast(src.c.inv_a, cc);
+  c.sv = bcast(src.c.sv, cc);
+  const int v2 = bcast_i(src.c.vl | (src.c.vr << 16), cc);
+  c.vl = v2 & 0xffff;
+  c.vr = int(unsigned(v2) >> 16);
+  c.ok = true;
+  lo = bcast_i(src.ulo, cc);
+  const int sj = bcast_i(src.span_j, cc);
+  span = sj & 0xffff;
+  J = int(unsigned(sj) >> 16);
 }
 
 __device__ __forceinline__ int wred_min(int x) {
@@ -813,6 +1008,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     // per-view constants of the column loop, read once (the loop's global
     // stores would otherwise force a reload per column)
     const double a12 = g.a12;
+    const T a12s = T(g.a12 * P.u_scl);
     const T w0 = T(g.w0), w1 = T(g.w1);
     const bool two = g.w1 != 0.0;
     const T *vol0 = vol + long(g.t0) * P.vox_per_phase;
@@ -849,6 +1045,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     if (any) {
       // lanes past the last slice work on a clamped k with a zero value
       const int kc = kin ? k : P.nz - 1;
+      const double kd = double(kc);
       // the voxels of the next live column are loaded while the current one
       // is projected (volume copy with k fastest: the 64 lanes read 64
       // consecutive words)
@@ -866,58 +1063,113 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         T x = w0 * xa;
         if (two) x += w1 * xb;
         if (pend) load_col(__builtin_ctzll(pend));
-        Col<T> c;
-        int lo, hi, np, i, j;
-        bcast_col<TAB>(cl, cc, c, lo, hi, np, i, j);
-        np = min(np, W);
-        const int span = hi - lo;
-        const int npv = c.vr - c.vl;
-        const bool use_prof = in_lds && span <= pmax && npv <= W;
-        T cv[W];
-        vweights<W, TAB>(P, c, lane, cv);
-        T val = T(0);
-        Vox<T> v;
-        bool act = false;
-        voxel<TAB>(P, a12, c, kc, v);
-        act = kin && on_detector(v);
-        val = kin ? x * v.rho : T(0);
-        if (use_prof) {
-          // prof[0, 128) is zero here (cleared at the start and after each
-          // outer product); longer spans clear the rest, spare entries included
-          for (int u = 128 + lane; u < span + W; u += 64) prof[u] = T(0);
-          __builtin_amdgcn_wave_barrier();
-          // Windows starting left of the detector use the W margin entries
-          // in front of the profile (never read).  Every lane runs the steps:
-          // lanes off the fast path add 0 into the margin, and the rare
-          // voxels that need the direct path take it in a separate block.
-          const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
-          profile_n<W, TAB>(P, np, v, fast ? prof + (v.ul - lo) : prof - W, fast ? val : T(0),
-                            plain);
-          if (__ballot(act && !fast)) {
-            T dummy = T(0);
-            if (act && !fast) slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
-          }
-          __builtin_amdgcn_wave_barrier();
-          // outer product P x cv; the lanes sit on distinct window columns and
-          // the rows of one column are distinct: one batch of reads, FMAs and
-          // writes per lane
-          T *dst0 = win + (c.vl - vl) * RW + (lo - ul);
-          if (span <= 128) {
-            // profile entries past the span may hold weights of pixels past
-            // the detector edge: masked to 0
-            const T p0 = lane < span ? prof[lane] : T(0);
-            const T p1 = lane + 64 < span ? prof[lane + 64] : T(0);
-            outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
+        if constexpr (TAB == 0) {
+          // symmetric windows: every voxel of the column takes the profile
+          // (its window lies inside the column profile by construction;
+          // entries off the detector are dropped at the outer product)
+          Col<T> c;
+          int lo, span, J;
+          bcast_sym(cl, cc, c, lo, span, J);
+          const int npv = c.vr - c.vl;
+          const bool use_prof = in_lds && span <= pmax && npv <= W && 2 * J + 3 <= W;
+          T cv[W];
+          vweights<W, 0>(P, c, lane, cv);
+          if (use_prof) {
+            // prof[0, 128) is zero here (cleared at the start and after each
+            // outer product); longer spans clear the rest
+            for (int u = 128 + lane; u < span; u += 64) prof[u] = T(0);
+            __builtin_amdgcn_wave_barrier();
+            Sym<T> sv;
+            sym_voxel(P, a12s, c, kd, lo, sv);
+            // lanes past the last slice share the clamped slice's window:
+            // they add their zeros into the W margin entries in front of the
+            // profile (never read), so no two lanes of a plain step meet
+            sym_profile_n<W>(J, sv, kin ? prof + (sv.p - J - 1) : prof - W,
+                             kin ? x * sv.rho : T(0), plain);
+            __builtin_amdgcn_wave_barrier();
+            // outer product P x cv over the column's pixels in the window
+            // [ul, ur) (which lies on the detector): pixel a + t is profile
+            // entry t0 + t
+            const int a = max(lo, ul), cnt = min(lo + span, ur) - a, t0 = a - lo;
+            T *dst0 = win + (c.vl - vl) * RW + (a - ul);
+            if (cnt <= 128) {
+              const T p0 = lane < cnt ? prof[t0 + lane] : T(0);
+              const T p1 = lane + 64 < cnt ? prof[t0 + lane + 64] : T(0);
+              outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
+            } else {
+              for (int u = lane; u < cnt; u += 64)
+                outer_n<W>(npv, dst0 + u, RW, cv, prof[t0 + u]);
+            }
+            __builtin_amdgcn_wave_barrier();
+            prof[lane] = T(0);
+            prof[lane + 64] = T(0);
+            __builtin_amdgcn_wave_barrier();
           } else {
-            for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
+            // direct path for the whole column
+            c.hu0 = bcast(cl.c.hu0, cc);
+            if (kin) {
+              Vox<T> v;
+              voxel(P, a12, c, kc, v);
+              if (on_detector(v)) {
+                T dummy = T(0);
+                slow_voxel<true, 0, T>(P, c, v, x * v.rho, out, nullptr, dummy);
+              }
+            }
           }
-          __builtin_amdgcn_wave_barrier();
-          prof[lane] = T(0);
-          prof[lane + 64] = T(0);
-          __builtin_amdgcn_wave_barrier();
-        } else if (act) {
-          T dummy = T(0);
-          slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
+        } else {
+          Col<T> c;
+          int lo, hi, np, i, j;
+          bcast_col<TAB>(cl, cc, c, lo, hi, np, i, j);
+          np = min(np, W);
+          const int span = hi - lo;
+          const int npv = c.vr - c.vl;
+          const bool use_prof = in_lds && span <= pmax && npv <= W;
+          T cv[W];
+          vweights<W, TAB>(P, c, lane, cv);
+          T val = T(0);
+          Vox<T> v;
+          bool act = false;
+          voxel<TAB>(P, a12, c, kc, v);
+          act = kin && on_detector(v);
+          val = kin ? x * v.rho : T(0);
+          if (use_prof) {
+            // prof[0, 128) is zero here (cleared at the start and after each
+            // outer product); longer spans clear the rest, spare entries included
+            for (int u = 128 + lane; u < span + W; u += 64) prof[u] = T(0);
+            __builtin_amdgcn_wave_barrier();
+            // Windows starting left of the detector use the W margin entries
+            // in front of the profile (never read).  Every lane runs the steps:
+            // lanes off the fast path add 0 into the margin, and the rare
+            // voxels that need the direct path take it in a separate block.
+            const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
+            profile_n<W, TAB>(P, np, v, fast ? prof + (v.ul - lo) : prof - W, fast ? val : T(0),
+                              plain);
+            if (__ballot(act && !fast)) {
+              T dummy = T(0);
+              if (act && !fast) slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // outer product P x cv; the lanes sit on distinct window columns and
+            // the rows of one column are distinct: one batch of reads, FMAs and
+            // writes per lane
+            T *dst0 = win + (c.vl - vl) * RW + (lo - ul);
+            if (span <= 128) {
+              // profile entries past the span may hold weights of pixels past
+              // the detector edge: masked to 0
+              const T p0 = lane < span ? prof[lane] : T(0);
+              const T p1 = lane + 64 < span ? prof[lane + 64] : T(0);
+              outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
+            } else {
+              for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
+            }
+            __builtin_amdgcn_wave_barrier();
+            prof[lane] = T(0);
+            prof[lane + 64] = T(0);
+            __builtin_amdgcn_wave_barrier();
+          } else if (act) {
+            T dummy = T(0);
+            slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
+          }
         }
       }
     }
@@ -1114,6 +1366,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       const ViewConst &g = P.views[view];
       const T *in = sino + long(view) * npix;
       const double a12 = g.a12;
+      const T a12s = T(g.a12 * P.u_scl);
       const T w0 = T(g.w0), w1 = T(g.w1), wsum = T(g.w0 + g.w1);
       // this view's columns sit in lanes loff .. loff + NCOL - 1
       const int loff = vslot * NCOL;
@@ -1162,74 +1415,140 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
 #pragma unroll 1
         for (int cc = 0; cc < NCOL; ++cc) {
           if (!((live_mask >> cc) & 1ull)) continue;
-          Col<T> c;
-          int lo, hi, np, i, j;
-          bcast_col<TAB>(cl, loff + cc, c, lo, hi, np, i, j);
-          np = min(np, W);
-          const int span = hi - lo;
-          const int npv = c.vr - c.vl;
-          const bool use_q = in_lds && span <= pmax && npv <= W;
-          T cv[W];
-          vweights<W, TAB>(P, c, lane, cv);
           T contrib[VPL];
 #pragma unroll
           for (int q = 0; q < VPL; ++q) contrib[q] = T(0);
-          if (use_q) {
-            const T *src0 = region + (c.vl - vl) * pitch + (lo - ul);
-            if (span <= 128 * VPL) {
-              // 128 pixels per batch of reads; lanes past the span read other
-              // entries of the LDS region (index < lds_cap + 128 VPL) and
-              // store zeros into the slack
+          if constexpr (TAB == 0) {
+            // symmetric windows: every voxel of the column gathers from the
+            // column's Q row (its window lies inside the row by construction)
+            Col<T> c;
+            int lo, span, J;
+            bcast_sym(cl, loff + cc, c, lo, span, J);
+            const int npv = c.vr - c.vl;
+            const bool use_q = in_lds && span <= pmax && npv <= W && 2 * J + 3 <= W;
+            T cv[W];
+            vweights<W, 0>(P, c, lane, cv);
+            if (use_q) {
+              // Q over the column profile: entry t is pixel lo + t, zero off
+              // the window [ul, ur) (which lies on the detector); pixel a + t
+              // is entry t0 + t
+              const int a = max(lo, ul), cnt = min(lo + span, ur) - a, t0 = a - lo;
+              const T *src0 = region + (c.vl - vl) * pitch + (a - ul);
+              T *qa = qrow + t0;
+              for (int u = lane; u < t0; u += 64) qrow[u] = T(0);
+              for (int u = t0 + cnt + lane; u < span; u += 64) qrow[u] = T(0);
+              if (cnt <= 128 * VPL) {
+                // 128 pixels per batch of reads; lanes past cnt read other
+                // entries of the LDS region (index < lds_cap + 128 VPL) and
+                // write nothing
 #pragma unroll
-              for (int h = 0; h < VPL; ++h) {
-                const int b = 128 * h;
-                if (h == 0 || span > b) {
-                  T q0, q1;
-                  if (fixed)
-                    q_n2c<W, kPitch>(npv, src0 + b + lane, cv, q0, q1);
-                  else
-                    q_n2<W>(npv, src0 + b + lane, RW, cv, q0, q1);
-                  if (b + lane < span + W) qrow[b + lane] = b + lane < span ? q0 : T(0);
-                  if (b + lane + 64 < span + W)
-                    qrow[b + lane + 64] = b + lane + 64 < span ? q1 : T(0);
+                for (int h = 0; h < VPL; ++h) {
+                  const int b = 128 * h;
+                  if (h == 0 || cnt > b) {
+                    T q0, q1;
+                    if (fixed)
+                      q_n2c<W, kPitch>(npv, src0 + b + lane, cv, q0, q1);
+                    else
+                      q_n2<W>(npv, src0 + b + lane, RW, cv, q0, q1);
+                    if (b + lane < cnt) qa[b + lane] = q0;
+                    if (b + lane + 64 < cnt) qa[b + lane + 64] = q1;
+                  }
+                }
+              } else {
+                for (int u = lane; u < cnt; u += 64) qa[u] = q_n<W>(npv, src0 + u, pitch, cv);
+              }
+              __builtin_amdgcn_wave_barrier();
+#pragma unroll
+              for (int q = 0; q < VPL; ++q) {
+                const int k = kf + kKT * q + lane;
+                const bool kin = k < P.nz;
+                Sym<T> sv;
+                sym_voxel(P, a12s, c, double(kin ? k : P.nz - 1), lo, sv);
+                const T gq = sym_gather_n<W>(J, sv, qrow + (sv.p - J - 1));
+                contrib[q] = kin ? gq * sv.rho : T(0);
+              }
+              __builtin_amdgcn_wave_barrier();
+            } else {
+              // direct path for the whole column
+              c.hu0 = bcast(cl.c.hu0, loff + cc);
+#pragma unroll
+              for (int q = 0; q < VPL; ++q) {
+                const int k = kf + kKT * q + lane;
+                if (k < P.nz) {
+                  Vox<T> v;
+                  voxel(P, a12, c, k, v);
+                  if (on_detector(v))
+                    slow_voxel<false, 0, T>(P, c, v, T(0), nullptr, in, contrib[q]);
                 }
               }
-            } else {
-              for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, pitch, cv);
             }
-            if (lane < W) qrow[span + lane] = T(0);
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int q = 0; q < VPL; ++q) {
-              const int k = kf + kKT * q + lane;
-              const bool kin = k < P.nz;
-              Vox<T> v;
-              voxel<TAB>(P, a12, c, kin ? k : P.nz - 1, v);
-              // All np pixels: zero weights past the footprint, zero Q past
-              // the column span (right detector edge) and in the W entries
-              // in front of the row (windows starting left of the detector).
-              // Every lane gathers: lanes off the fast path read the zero
-              // entries, and the rare voxels that need the direct path take it
-              // in a separate block.
-              const bool act = kin && on_detector(v);
-              const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
-              const T gq = gather_n<W, TAB>(P, np, v, fast ? qrow + (v.ul - lo) : qrow - W);
-              contrib[q] = fast ? gq * v.rho : T(0);
-              if (__ballot(act && !fast)) {
-                if (act && !fast)
-                  slow_voxel<false, TAB, T>(P, c, v, T(0), nullptr, in, contrib[q]);
-              }
-            }
-            __builtin_amdgcn_wave_barrier();
           } else {
-#pragma unroll
-            for (int q = 0; q < VPL; ++q) {
-              const int k = kf + kKT * q + lane;
-              if (k < P.nz) {
+            Col<T> c;
+            int lo, hi, np, i, j;
+            bcast_col<TAB>(cl, loff + cc, c, lo, hi, np, i, j);
+            np = min(np, W);
+            const int span = hi - lo;
+            const int npv = c.vr - c.vl;
+            const bool use_q = in_lds && span <= pmax && npv <= W;
+            T cv[W];
+            vweights<W, TAB>(P, c, lane, cv);
+            if (use_q) {
+              const T *src0 = region + (c.vl - vl) * pitch + (lo - ul);
+              if (span <= 128 * VPL) {
+                // 128 pixels per batch of reads; lanes past the span read other
+                // entries of the LDS region (index < lds_cap + 128 VPL) and
+                // store zeros into the slack
+  #pragma unroll
+                for (int h = 0; h < VPL; ++h) {
+                  const int b = 128 * h;
+                  if (h == 0 || span > b) {
+                    T q0, q1;
+                    if (fixed)
+                      q_n2c<W, kPitch>(npv, src0 + b + lane, cv, q0, q1);
+                    else
+                      q_n2<W>(npv, src0 + b + lane, RW, cv, q0, q1);
+                    if (b + lane < span + W) qrow[b + lane] = b + lane < span ? q0 : T(0);
+                    if (b + lane + 64 < span + W)
+                      qrow[b + lane + 64] = b + lane + 64 < span ? q1 : T(0);
+                  }
+                }
+              } else {
+                for (int u = lane; u < span; u += 64) qrow[u] = q_n<W>(npv, src0 + u, pitch, cv);
+              }
+              if (lane < W) qrow[span + lane] = T(0);
+              __builtin_amdgcn_wave_barrier();
+  #pragma unroll
+              for (int q = 0; q < VPL; ++q) {
+                const int k = kf + kKT * q + lane;
+                const bool kin = k < P.nz;
                 Vox<T> v;
-                voxel<TAB>(P, a12, c, k, v);
-                if (on_detector(v))
-                  slow_voxel<false, TAB, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+                voxel<TAB>(P, a12, c, kin ? k : P.nz - 1, v);
+                // All np pixels: zero weights past the footprint, zero Q past
+                // the column span (right detector edge) and in the W entries
+                // in front of the row (windows starting left of the detector).
+                // Every lane gathers: lanes off the fast path read the zero
+                // entries, and the rare voxels that need the direct path take it
+                // in a separate block.
+                const bool act = kin && on_detector(v);
+                const bool fast = act && v.ul >= lo - W && v.ur <= hi && fits(v, np, P.nu);
+                const T gq = gather_n<W, TAB>(P, np, v, fast ? qrow + (v.ul - lo) : qrow - W);
+                contrib[q] = fast ? gq * v.rho : T(0);
+                if (__ballot(act && !fast)) {
+                  if (act && !fast)
+                    slow_voxel<false, TAB, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+                }
+              }
+              __builtin_amdgcn_wave_barrier();
+            } else {
+  #pragma unroll
+              for (int q = 0; q < VPL; ++q) {
+                const int k = kf + kKT * q + lane;
+                if (k < P.nz) {
+                  Vox<T> v;
+                  voxel<TAB>(P, a12, c, k, v);
+                  if (on_detector(v))
+                    slow_voxel<false, TAB, T>(P, c, v, T(0), nullptr, in, contrib[q]);
+                }
               }
             }
           }
