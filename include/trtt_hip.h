@@ -295,6 +295,11 @@ typedef struct trtt_hip_tuning {
                               projections with global atomic adds; 0: windows go to
                               scratch slots and a reduction kernel sums them (on a
                               second stream, overlapped with the next chunk) */
+  int fwd_xcds;            /* 0 or 1: forward grid of tiles x view groups; n > 1 (atomic
+                              flush): a 1-D grid in an XCD-aware order for a device that
+                              deals consecutive workgroups to n XCDs in turn, so that all
+                              tiles of one view group and k block run on one XCD and
+                              their atomic adds stay in its L2 (MI355X: 8) */
 } trtt_hip_tuning_t;
 int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *plan, trtt_hip_tuning_t *tuning);
 int trtt_hip_plan_set_tuning(trtt_hip_plan_t *plan, const trtt_hip_tuning_t *tuning);

@@ -864,11 +864,14 @@ def test_axial_forward_tiles_vs_oracle(lib, oracle, nt, model):
                     _tab(model))
     xd = _dev(x.reshape(P.vol_shape), torch.float32)
     for tile in sorted(AT.FWD_TILES):
-        for wb, vpb, flush in [(24576, 0, 1), (16384, 3, 0), (16384, 3, 1), (4096, 1, 1)]:
+        # fwd_xcds: XCD-aware block order (8: MI355X; 3: a count that leaves
+        # padding blocks in the 1-D grid)
+        for wb, vpb, flush, xcds in [(24576, 0, 1, 0), (16384, 3, 0, 0), (16384, 3, 1, 0),
+                                     (4096, 1, 1, 0), (8192, 4, 1, 8), (16384, 3, 1, 3)]:
             AT.set_tuning(P, fwd_tile=tile, fwd_window_bytes=wb, fwd_views_per_block=vpb,
-                          fwd_flush=flush)
+                          fwd_flush=flush, fwd_xcds=xcds)
             fwd = P.forward(xd).double().cpu().numpy()
-            assert rel_err(fwd, rf) < 1e-5, (tile, wb, vpb, flush)
+            assert rel_err(fwd, rf) < 1e-5, (tile, wb, vpb, flush, xcds)
     assert P.error_flag() == 0
     P.close()
 

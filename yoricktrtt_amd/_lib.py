@@ -29,7 +29,7 @@ class Tuning(C.Structure):
 
     _fields_ = [("fwd_tile", C.c_int), ("bwd_tile", C.c_int), ("fwd_window_bytes", C.c_int),
                 ("fwd_views_per_block", C.c_int), ("bwd_window_bytes", C.c_int),
-                ("fwd_chunk_views", C.c_int), ("fwd_flush", C.c_int)]
+                ("fwd_chunk_views", C.c_int), ("fwd_flush", C.c_int), ("fwd_xcds", C.c_int)]
 
 
 class LF2DParameters(C.Structure):

@@ -34,6 +34,9 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kKT = 64;  // k per tile (one per lane)
 // fixed row pitch of the backward's staged window (one row per thread)
 constexpr int kPitch = kThreads;
+// fixed row pitch of the forward's detector window
+constexpr int kFwdPitch = 128;
+
 
 // F(x) - 1/2 for the cubic B-spline CDF: sign(x) G(min(|x|, 2)) with, on
 // [0, 2], G(t) = 1/2 - (2-t)^4/24 + max(1-t, 0)^4/6 (the two pieces
@@ -360,19 +363,45 @@ __device__ __forceinline__ T sym_cdf(T bs, T br) {
   return fma(s2, s2, -(r2 * r2));
 }
 
+// The r term of bound j vanishes once j >= hu: its base is
+// (1 - (f + j)/hu)/6^(1/4) on the left and (1 - (1 - f + j)/hu)/6^(1/4) on
+// the right, with f in [0, 1).  The column's J = floor(hs_max) with
+// hs = 2 hu (the closed-form cubic, the only footprint of this path) gives
+// floor(hu_max) = floor(J/2), so every bound j >= J/2 + 1 lies past hu for
+// every voxel of the column and takes F = s^4 alone (three instructions).
+template <typename T>
+__device__ __forceinline__ T sym_cdf_s(T bs) {
+  const T s = fmin(fmax(bs, T(0)), T(1));
+  const T s2 = s * s;
+  return s2 * s2;
+}
+
+// CDF values at the bounds: L[j] = F at left bound j, R[j] = 1 - F at
+// right bound j.
+template <int J, typename T>
+__device__ __forceinline__ void sym_bounds(const Sym<T> &s, T (&L)[J + 1], T (&R)[J + 1]) {
+  constexpr int JR = J / 2 + 1;  // first bound without the r term
+#pragma unroll
+  for (int j = 0; j <= J; ++j) {
+    const T bl = j == 0 ? s.bl : fma(T(-j), s.A, s.bl);
+    const T br = j == 0 ? s.br : fma(T(-j), s.A, s.br);
+    if (j < JR) {
+      const T rl = j == 0 ? s.rl : fma(T(-j), s.A2, s.rl);
+      const T rr = j == 0 ? s.rr : fma(T(-j), s.A2, s.rr);
+      L[j] = sym_cdf(bl, rl);
+      R[j] = sym_cdf(br, rr);
+    } else {
+      L[j] = sym_cdf_s(bl);
+      R[j] = sym_cdf_s(br);
+    }
+  }
+}
+
 // Weights of the pixels fl - J - 1 + t, t < 2J + 3.
 template <int J, typename T>
 __device__ __forceinline__ void sym_weights(const Sym<T> &s, T (&w)[2 * J + 3]) {
   T L[J + 1], R[J + 1];
-#pragma unroll
-  for (int j = 0; j <= J; ++j) {
-    const T bl = j == 0 ? s.bl : fma(T(-j), s.A, s.bl);
-    const T rl = j == 0 ? s.rl : fma(T(-j), s.A2, s.rl);
-    const T br = j == 0 ? s.br : fma(T(-j), s.A, s.br);
-    const T rr = j == 0 ? s.rr : fma(T(-j), s.A2, s.rr);
-    L[j] = sym_cdf(bl, rl);
-    R[j] = sym_cdf(br, rr);
-  }
+  sym_bounds<J>(s, L, R);
   w[0] = L[J];
 #pragma unroll
   for (int j = 0; j < J; ++j) w[J - j] = L[j] - L[j + 1];
@@ -530,6 +559,38 @@ __device__ __forceinline__ void outer_n2(int n, T *dst, int RW, const T (&cv)[W]
   }
 }
 
+// outer_rows2 with a compile-time row pitch: the row offsets become
+// immediate offsets of the LDS reads and writes.
+template <int N, int PITCH, typename T>
+__device__ __forceinline__ void outer_rows2c(T *dst, const T *cv, T p0, T p1) {
+  T t0[N], t1[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    t0[m] = dst[m * PITCH];
+    t1[m] = dst[m * PITCH + 64];
+  }
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    dst[m * PITCH] = t0[m] + cv[m] * p0;
+    dst[m * PITCH + 64] = t1[m] + cv[m] * p1;
+  }
+}
+
+template <int W, int PITCH, typename T>
+__device__ __forceinline__ void outer_n2c(int n, T *dst, const T (&cv)[W], T p0, T p1) {
+  switch (n) {
+#define TRTT_OUTER2C(NN) \
+  case NN:               \
+    if (NN <= W) outer_rows2c<(NN <= W ? NN : 1), PITCH, T>(dst, cv, p0, p1); \
+    break;
+    TRTT_OUTER2C(1) TRTT_OUTER2C(2) TRTT_OUTER2C(3) TRTT_OUTER2C(4) TRTT_OUTER2C(5)
+    TRTT_OUTER2C(6) TRTT_OUTER2C(7) TRTT_OUTER2C(8) TRTT_OUTER2C(9) TRTT_OUTER2C(10)
+    TRTT_OUTER2C(11) TRTT_OUTER2C(12) TRTT_OUTER2C(13)
+#undef TRTT_OUTER2C
+    default: break;
+  }
+}
+
 template <int W, typename T>
 __device__ __forceinline__ void outer_n(int n, T *dst, int RW, const T (&cv)[W], T pu) {
   switch (n) {
@@ -647,13 +708,13 @@ __device__ __forceinline__ void sym_profile_n(int J, const Sym<T> &s, T *pp, T v
 template <int W, typename T>
 __device__ __forceinline__ T sym_gather_n(int J, const Sym<T> &s, const T *qq) {
   switch (J) {
-#define TRTT_SG(JJ)                    \
-  case JJ:                             \
-    if constexpr (2 * JJ + 3 <= W) {   \
-      T w[2 * JJ + 3];                 \
-      sym_weights<JJ>(s, w);           \
+#define TRTT_SG(JJ)                        \
+  case JJ:                                 \
+    if constexpr (2 * JJ + 3 <= W) {       \
+      T w[2 * JJ + 3];                     \
+      sym_weights<JJ>(s, w);               \
       return dot_steps<2 * JJ + 3>(w, qq); \
-    }                                  \
+    }                                      \
     break;
     TRTT_SG(0) TRTT_SG(1) TRTT_SG(2) TRTT_SG(3) TRTT_SG(4) TRTT_SG(5)
 #undef TRTT_SG
@@ -731,9 +792,8 @@ __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T
 
 struct TileMap {
   int bx, by, bz;
-  __device__ __forceinline__ void init(const DevPlan &P, int cx, int cy) {
+  __device__ __forceinline__ void init(const DevPlan &P, int cx, int cy, int b) {
     const int ntx = (P.nx + cx - 1) / cx, nty = (P.ny + cy - 1) / cy;
-    int b = blockIdx.x;
     bx = b % ntx;
     b /= ntx;
     by = b % nty;
@@ -964,7 +1024,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
                                           T *__restrict__ sino, int views_per_block, int wcap,
                                           int pmax, int view_base, int view_end,
                                           T *__restrict__ scratch, int4 *__restrict__ hdr,
-                                          int plain, int flush) {
+                                          int plain, int flush, int ntiles, int xcds) {
   constexpr int NCOL = CX * CY;  // columns of the wave
   static_assert(NCOL <= 64, "one column per lane");
   // With scratch windows (flush 0) the previous chunk's reduction runs on
@@ -976,20 +1036,36 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
   const int lane = threadIdx.x;
   T *win = reinterpret_cast<T *>(g_smem);
   T *prof = win + wcap + W;  // after W margin entries
+  // block -> (tile, view group).  Grid (tiles, groups) by default.  With
+  // xcds > 1 the grid is 1-D in an XCD-aware order: the hardware deals
+  // consecutive workgroups to the XCDs in turn, so block L runs on XCD
+  // L mod xcds.  A unit is one view group on one k block of tiles; XCD x
+  // runs the units x, x + xcds, ..., every tile of a unit on the same XCD,
+  // so the detector lines that a unit's atomic adds touch stay in one L2.
+  int tile = int(blockIdx.x), group = int(blockIdx.y);
+  if (xcds > 1) {
+    const int nbz = (P.nz + kKT - 1) / kKT;
+    const int txy = ntiles / nbz;
+    const int L = int(blockIdx.x);
+    const int sq = L / xcds;
+    const int u = (L - sq * xcds) + xcds * (sq / txy);
+    group = u / nbz;
+    tile = sq % txy + txy * (u - group * nbz);
+  }
+  const int first_view = view_base + group * views_per_block;
+  if (first_view >= view_end) return;  // padding of the XCD-aware grid
   TileMap tm;
-  tm.init(P, CX, CY);
+  tm.init(P, CX, CY, tile);
   const int k = tm.bz * kKT + lane;
   const bool kin = k < P.nz;
   const int kf = tm.bz * kKT, kl = min(kf + kKT - 1, P.nz - 1);
   const long npix = long(P.nu) * P.nv;
-  const long ntiles = gridDim.x;
   prof[lane] = T(0);
   prof[lane + 64] = T(0);
 
   // When the column tables of all views of the block fit in the wave, they
   // are evaluated at once: lane vv NCOL + c holds column c on view vv.
   const bool multi = views_per_block * NCOL <= 64;
-  const int first_view = view_base + int(blockIdx.y) * views_per_block;
   ColLane<T> cl;
   if (multi) {
     const int slot = lane / NCOL;
@@ -1004,7 +1080,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     if (view >= view_end) break;
     const ViewConst &g = P.views[view];
     T *out = sino + long(view) * npix;
-    const long slot = long(view - view_base) * ntiles + blockIdx.x;
+    const long slot = long(view - view_base) * ntiles + tile;
     // per-view constants of the column loop, read once (the loop's global
     // stores would otherwise force a reload per column)
     const double a12 = g.a12;
@@ -1031,11 +1107,16 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     const bool any = ur > ul && vr > vl;
     const int RW = any ? (ur - ul + 3) & ~3 : 0, RH = any ? vr - vl : 0;
     const bool in_lds = any && RW * RH <= wcap;
+    // Window rows at the fixed pitch kFwdPitch where they fit (atomic flush
+    // only: a scratch window keeps the RW layout its header describes); the
+    // outer product then reaches its rows through immediate LDS offsets.
+    const bool fixed = in_lds && flush && RW <= kFwdPitch && kFwdPitch * RH <= wcap;
+    const int pitch = fixed ? kFwdPitch : RW;
     // window zeroing and flush in 16-byte vectors: the capacity wcap is a
     // multiple of 64 elements, so the rounded-up count stays inside it
     using V16 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
     constexpr int kE = 16 / int(sizeof(T));
-    const int nvec = (RW * RH + kE - 1) / kE;
+    const int nvec = (pitch * RH + kE - 1) / kE;
     if (in_lds) {
       V16 *w16 = reinterpret_cast<V16 *>(win);
       for (int q = lane; q < nvec; q += 64) w16[q] = V16{};
@@ -1091,14 +1172,17 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
             // [ul, ur) (which lies on the detector): pixel a + t is profile
             // entry t0 + t
             const int a = max(lo, ul), cnt = min(lo + span, ur) - a, t0 = a - lo;
-            T *dst0 = win + (c.vl - vl) * RW + (a - ul);
+            T *dst0 = win + (c.vl - vl) * pitch + (a - ul);
             if (cnt <= 128) {
               const T p0 = lane < cnt ? prof[t0 + lane] : T(0);
               const T p1 = lane + 64 < cnt ? prof[t0 + lane + 64] : T(0);
-              outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
+              if (fixed)
+                outer_n2c<W, kFwdPitch>(npv, dst0 + lane, cv, p0, p1);
+              else
+                outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
             } else {
               for (int u = lane; u < cnt; u += 64)
-                outer_n<W>(npv, dst0 + u, RW, cv, prof[t0 + u]);
+                outer_n<W>(npv, dst0 + u, pitch, cv, prof[t0 + u]);
             }
             __builtin_amdgcn_wave_barrier();
             prof[lane] = T(0);
@@ -1152,15 +1236,18 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
             // outer product P x cv; the lanes sit on distinct window columns and
             // the rows of one column are distinct: one batch of reads, FMAs and
             // writes per lane
-            T *dst0 = win + (c.vl - vl) * RW + (lo - ul);
+            T *dst0 = win + (c.vl - vl) * pitch + (lo - ul);
             if (span <= 128) {
               // profile entries past the span may hold weights of pixels past
               // the detector edge: masked to 0
               const T p0 = lane < span ? prof[lane] : T(0);
               const T p1 = lane + 64 < span ? prof[lane + 64] : T(0);
-              outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
+              if (fixed)
+                outer_n2c<W, kFwdPitch>(npv, dst0 + lane, cv, p0, p1);
+              else
+                outer_n2<W>(npv, dst0 + lane, RW, cv, p0, p1);
             } else {
-              for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, RW, cv, prof[u]);
+              for (int u = lane; u < span; u += 64) outer_n<W>(npv, dst0 + u, pitch, cv, prof[u]);
             }
             __builtin_amdgcn_wave_barrier();
             prof[lane] = T(0);
@@ -1183,14 +1270,14 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       if (cw <= 128) {
         const bool c0 = lane < cw, c1 = lane + 64 < cw;
         for (int r = 0; r < RH; ++r, orow += P.nu) {
-          const T a = win[r * RW + lane], b = win[r * RW + lane + 64];
+          const T a = win[r * pitch + lane], b = win[r * pitch + lane + 64];
           if (c0 && a != T(0)) unsafeAtomicAdd(orow + lane, a);
           if (c1 && b != T(0)) unsafeAtomicAdd(orow + lane + 64, b);
         }
       } else {
         for (int r = 0; r < RH; ++r, orow += P.nu)
           for (int cI = lane; cI < cw; cI += 64) {
-            const T a = win[r * RW + cI];
+            const T a = win[r * pitch + cI];
             if (a != T(0)) unsafeAtomicAdd(orow + cI, a);
           }
       }
@@ -1315,6 +1402,13 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(DevPlan P, const T *__res
 // ---------------------------------------------------------------------------
 // Backward
 // ---------------------------------------------------------------------------
+// The accumulators of a lane are one vector register tuple: the column loop
+// (not unrolled) indexes them with its wave-uniform counter, which indexed
+// VGPR moves serve (an array indexed that way is placed in scratch memory,
+// a memory round trip per column).
+template <typename T, int N>
+using AccVec = T __attribute__((ext_vector_type(N)));
+
 // VPL voxels per lane: the tile spans 64 VPL slices and lane L holds the
 // voxels k = kf + 64 q + L, q < VPL, so that the per-column set-up (column
 // broadcast, v weights, Q row) serves 64 VPL voxels.
@@ -1333,7 +1427,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
   if (lane < W) qrow[lane - W] = T(0);
   __shared__ int red[4 * kWaves];
   TileMap tm;
-  tm.init(P, CX, CY);
+  tm.init(P, CX, CY, blockIdx.x);
   const int kf = tm.bz * (kKT * VPL), kl = min(kf + kKT * VPL - 1, P.nz - 1);
   const long npix = long(P.nu) * P.nv;
   const int s_begin = seg_begin[blockIdx.y], s_end = seg_begin[blockIdx.y + 1];
@@ -1341,11 +1435,8 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
   for (int s = s_begin; s < s_end; ++s) {
     const Segment seg = P.segs[s];
     // TWO: some segment interpolates between two phases (second accumulator)
-    T acc0[NA], acc1[TWO ? NA : 1];
-#pragma unroll
-    for (int a = 0; a < NA; ++a) acc0[a] = T(0);
-#pragma unroll
-    for (int a = 0; a < (TWO ? NA : 1); ++a) acc1[a] = T(0);
+    AccVec<T, NA> acc0 = T(0);
+    AccVec<T, TWO ? NA : 1> acc1 = T(0);
 
     // The column tables of NVT consecutive views of the segment are
     // evaluated at once: lane v NCOL + c holds column c on view n0 + v.
@@ -1558,7 +1649,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
               acc0[cc * VPL + q] += wsum * contrib[q];
             } else {
               acc0[cc * VPL + q] += w0 * contrib[q];
-              acc1[TWO ? cc * VPL + q : 0] += w1 * contrib[q];
+              if constexpr (TWO) acc1[cc * VPL + q] += w1 * contrib[q];
             }
           }
         }
@@ -1635,10 +1726,10 @@ size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
 template <bool CONE, int W, int CX, int CY, typename T>
 void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *volk, T *out,
                 int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain,
-                int flush) {
+                int flush, int ntiles, int xcds) {
 #define TRTT_FWD(TB)                                                                           \
   hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, TB, T>), grid, dim3(64), lds_bytes, st, P, volk, \
-                     out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush)
+                     out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush, ntiles, xcds)
   if (!P.table)
     TRTT_FWD(0);
   else if (P.table_deg == 2)
@@ -1690,7 +1781,17 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
     const size_t hdr_bytes = axial_hdr_bytes(P, chunk, tu);
     for (int c = 0, vb = 0; vb < P.nviews; vb += chunk, ++c) {
       const int ve = vb + chunk < P.nviews ? vb + chunk : P.nviews;
-      dim3 grid(ntiles, (ve - vb + views_per_block - 1) / views_per_block);
+      const int ngroups = (ve - vb + views_per_block - 1) / views_per_block;
+      dim3 grid(ntiles, ngroups);
+      // XCD-aware order (atomic flush): units of one view group and one k
+      // block of tiles, dealt to the XCDs in turn; blocks past the last unit
+      // return at once
+      const int xcds = flush && tu.fwd_xcds > 1 ? tu.fwd_xcds : 0;
+      if (xcds) {
+        const long nbz = (P.nz + axial::kKT - 1) / axial::kKT;
+        const long units = long(ngroups) * nbz;
+        grid = dim3(unsigned(long(xcds) * (long(ntiles) / nbz) * ((units + xcds - 1) / xcds)));
+      }
       const int b = overlap ? (c & 1) : 0;
       auto *o = static_cast<T *>(out);
       auto *sc = static_cast<T *>(scratch) + b * win_elems;
@@ -1701,15 +1802,15 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
       switch (tu.fwd_tile) {
         case 1:
           launch_fwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain, flush);
+                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds);
           break;
         case 2:
           launch_fwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain, flush);
+                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds);
           break;
         default:
           launch_fwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain, flush);
+                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds);
       }
       if (ev) hipEventRecord(ev[2 * npair + 1], st);
       ++npair;

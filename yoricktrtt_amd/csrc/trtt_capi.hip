@@ -855,6 +855,7 @@ int trtt_hip_plan_get_tuning(const trtt_hip_plan_t *p, trtt_hip_tuning_t *t) {
   t->bwd_window_bytes = p->tune.bwd_window_bytes;
   t->fwd_chunk_views = p->tune.fwd_chunk_views;
   t->fwd_flush = p->tune.fwd_flush;
+  t->fwd_xcds = p->tune.fwd_xcds;
   return TRTT_SUCCESS;
 }
 
@@ -871,6 +872,7 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
     return fail("backward window must be 4096..131072 bytes, a multiple of 256");
   if (t->fwd_chunk_views < 0) return fail("views per forward chunk must be >= 0");
   if (t->fwd_flush != 0 && t->fwd_flush != 1) return fail("forward flush mode must be 0 or 1");
+  if (t->fwd_xcds < 0 || t->fwd_xcds > 64) return fail("forward XCD count must be 0..64");
   if (p->axial_w <= 0) return TRTT_SUCCESS;  // general kernels: nothing to tune
   if (trtt::bwd_tile_vpl(t->bwd_tile) == 2 && p->axial_pmax2 == 0)
     return fail("backward tiles of 128 slices need column profiles of at most 256 pixels");
@@ -884,6 +886,7 @@ int trtt_hip_plan_set_tuning(trtt_hip_plan_t *p, const trtt_hip_tuning_t *t) {
   p->tune.bwd_window_bytes = t->bwd_window_bytes;
   p->tune.fwd_chunk_views = t->fwd_chunk_views;
   p->tune.fwd_flush = t->fwd_flush;
+  p->tune.fwd_xcds = t->fwd_xcds;
   if (old.fwd_tile != p->tune.fwd_tile || old.fwd_window_bytes != p->tune.fwd_window_bytes ||
       old.fwd_chunk_views != p->tune.fwd_chunk_views || old.fwd_flush != p->tune.fwd_flush) {
     // hipFree waits for the device: no launch of this plan is in flight

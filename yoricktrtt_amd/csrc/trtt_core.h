@@ -104,6 +104,7 @@ struct AxialTuning {
   int fwd_chunk_views = 0;        // views per forward scratch round, 0: fill the budget
   int fwd_flush = 1;              // 1: windows added to the projections with global atomics;
                                   // 0: written to scratch slots, summed by fwd_reduce
+  int fwd_xcds = 0;               // > 1: XCD-aware forward block order for that many XCDs
 };
 
 // Second stream of the axial forward: the reduction of view chunk c runs on
