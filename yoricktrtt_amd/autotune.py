@@ -28,7 +28,7 @@ FWD_TILES = {0: "8x4", 1: "4x4", 2: "8x8"}
 BWD_TILES = {0: "8x8", 1: "8x4", 2: "4x4", 3: "8x4x128", 4: "4x4x128", 5: "8x8x128"}
 # LDS window sizes tried per forward tile (a window too small for a tile's
 # detector footprint sends its voxels down the slow global-atomic path)
-WINDOWS = {0: (12288, 16384, 24576), 1: (8192, 12288, 16384), 2: (24576, 32768)}
+WINDOWS = {0: (12288, 16384, 24576), 1: (7168, 7424, 8192, 12288), 2: (24576, 32768)}
 VPBS = (0, 1, 2, 3, 4)
 CHUNKS = (0, 2, 4)  # views per forward scratch round (0: as many as fit the budget)
 BWD_WINDOWS = {0: (16384, 24576, 32768), 1: (12288, 16384, 24576), 2: (8192, 12288, 16384),

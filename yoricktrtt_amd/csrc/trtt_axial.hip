@@ -332,18 +332,18 @@ struct Sym {
   int p;              // centre pixel fl relative to the column profile start
 };
 
-// a12s = a12 u_scl; lo: first pixel of the column profile.  The distortion
+// uscl = u_scl, a12s = a12 u_scl; lo: first pixel of the column profile.  The distortion
 // factor takes the centre in single precision (its relative error reaches
 // du only through (uc - a12)^2 / a, far below single precision); the
 // sub-pixel offset f comes from the FP64 centre.
 template <typename T>
-__device__ __forceinline__ void sym_voxel(const DevPlan &P, T a12s, const Col<T> &c, double kd,
-                                          int lo, Sym<T> &s) {
+__device__ __forceinline__ void sym_voxel(T uscl, T a12s, const Col<T> &c, double kd, int lo,
+                                          Sym<T> &s) {
   const double uc = col_uc(c, kd);
   const double fl = floor(uc);
   const T f = T(uc - fl);
   s.p = int(fl) - lo;
-  const T uca = fma(T(uc), T(P.u_scl), -a12s);
+  const T uca = fma(T(uc), uscl, -a12s);
   const T dsq = fma(uca * c.inv_a, uca, T(1));
   const T rs = frsq(dsq);  // 1 / du
   s.A = c.ca * rs;
@@ -421,7 +421,9 @@ __device__ __forceinline__ float from_next_lane(float v) {
 
 // v weights of the column: lane m evaluates the CDF at the m-th v pixel
 // bound, one shifted difference gives the weight of pixel m in lane m (zero
-// from the window's end on), and readlane broadcasts the W weights.
+// from the window's end on), and readlane broadcasts the W weights.  (An
+// LDS slot per wave read back with broadcast reads instead was slower in
+// both kernels: the backward's extra LDS costs a workgroup per CU.)
 template <int W, int TAB = 0, typename T>
 __device__ __forceinline__ void vweights(const DevPlan &P, const Col<T> &c, int lane, T (&cv)[W]) {
   const T f = TAB ? table_cdf<TAB, T>(P, fma(double(lane), c.sdv, c.xv0))
@@ -432,6 +434,7 @@ __device__ __forceinline__ void vweights(const DevPlan &P, const Col<T> &c, int 
 #pragma unroll
   for (int m = 0; m < W; ++m) cv[m] = bcast(d, m);
 }
+
 
 // Row loops with a wave-uniform row count N (1..W), dispatched once per
 // column so the rows run without per-row branches.
@@ -988,6 +991,19 @@ __device__ __forceinline__ void block_window(const ColLane<T> &cl, bool live, in
   block_union(ul, ur, vl, vr, red);
 }
 
+// Detector window of the columns held by the lanes where m is true (wave-
+// uniform results): it starts on a multiple of 4 pixels and its rows are a
+// multiple of 4 long; true when it is not empty and fits cap elements.
+template <typename T>
+__device__ __forceinline__ bool window_fits(const ColLane<T> &cl, bool m, int cap, int &ul, int &ur,
+                                            int &vl, int &vr) {
+  ul = bcast_i(wred_min(m ? cl.lo : INT_MAX), 0) & ~3;
+  ur = bcast_i(wred_max(m ? cl.hi : INT_MIN), 0);
+  vl = bcast_i(wred_min(m ? cl.c.vl : INT_MAX), 0);
+  vr = bcast_i(wred_max(m ? cl.c.vr : INT_MIN), 0);
+  return ur > ul && vr > vl && ((ur - ul + 3) & ~3) * (vr - vl) <= cap;
+}
+
 // ---------------------------------------------------------------------------
 // Forward.  One wave per workgroup: the wave owns a tile of CX x CY columns
 // x 64 k and a private detector window in LDS, so no barrier or atomic is
@@ -1036,6 +1052,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
   const int lane = threadIdx.x;
   T *win = reinterpret_cast<T *>(g_smem);
   T *prof = win + wcap + W;  // after W margin entries
+  const T uscl = T(P.u_scl);
   // block -> (tile, view group).  Grid (tiles, groups) by default.  With
   // xcds > 1 the grid is 1-D in an XCD-aware order: the hardware deals
   // consecutive workgroups to the XCDs in turn, so block L runs on XCD
@@ -1096,17 +1113,36 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     }
     // this view's columns sit in lanes loff .. loff + NCOL - 1
     const int loff = multi ? vv * NCOL : 0;
-    const bool live = cl.live && lane >= loff && lane < loff + NCOL;
-    // wave-uniform (readfirstlane: kept in scalar registers)
-    // the window starts on a multiple of 4 pixels and its rows are a
-    // multiple of 4 long (16-byte loads in the reduction)
-    const int ul = bcast_i(wred_min(live ? cl.lo : INT_MAX), 0) & ~3;
-    const int ur = bcast_i(wred_max(live ? cl.hi : INT_MIN), 0);
-    const int vl = bcast_i(wred_min(live ? cl.c.vl : INT_MAX), 0);
-    const int vr = bcast_i(wred_max(live ? cl.c.vr : INT_MIN), 0);
+    const bool inview = cl.live && lane >= loff && lane < loff + NCOL;
+    // The columns of the view run in one pass with one window, or, when
+    // that window does not fit the LDS capacity (atomic flush only: a
+    // scratch window has one slot per view and tile), in two passes over
+    // two halves of the tile (its rows of columns split, else its columns
+    // of columns), each with its own window and flush.  A view whose halves
+    // do not fit either takes the direct path.
+    const int cidx = lane - loff;
+    const bool half_rows = cidx / CX < CY / 2, half_cols = cidx % CX < CX / 2;
+    int split = 0;
+    // window of the whole view (wave-uniform: kept in scalar registers)
+    int ul0, ur0, vl0, vr0;
+    const bool fits0 = window_fits(cl, inview, wcap, ul0, ur0, vl0, vr0);
+    if (flush && !fits0 && ur0 > ul0 && vr0 > vl0) {
+      int a, b, c, d, e, f, g, h;
+      if (window_fits(cl, inview && half_rows, wcap, a, b, c, d) &&
+          window_fits(cl, inview && !half_rows, wcap, e, f, g, h))
+        split = 1;
+      else if (window_fits(cl, inview && half_cols, wcap, a, b, c, d) &&
+               window_fits(cl, inview && !half_cols, wcap, e, f, g, h))
+        split = 2;
+    }
+    for (int pass = 0; pass < (split ? 2 : 1); ++pass) {
+    const bool live =
+        inview && (split == 0 || (pass == 0) == (split == 1 ? half_rows : half_cols));
+    int ul = ul0, ur = ur0, vl = vl0, vr = vr0;
+    bool in_lds = fits0;
+    if (split) in_lds = window_fits(cl, live, wcap, ul, ur, vl, vr);
     const bool any = ur > ul && vr > vl;
     const int RW = any ? (ur - ul + 3) & ~3 : 0, RH = any ? vr - vl : 0;
-    const bool in_lds = any && RW * RH <= wcap;
     // Window rows at the fixed pitch kFwdPitch where they fit (atomic flush
     // only: a scratch window keeps the RW layout its header describes); the
     // outer product then reaches its rows through immediate LDS offsets.
@@ -1161,7 +1197,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
             for (int u = 128 + lane; u < span; u += 64) prof[u] = T(0);
             __builtin_amdgcn_wave_barrier();
             Sym<T> sv;
-            sym_voxel(P, a12s, c, kd, lo, sv);
+            sym_voxel(uscl, a12s, c, kd, lo, sv);
             // lanes past the last slice share the clamped slice's window:
             // they add their zeros into the W margin entries in front of the
             // profile (never read), so no two lanes of a plain step meet
@@ -1288,6 +1324,7 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     }
     if (lane == 0 && !flush) hdr[slot] = in_lds ? make_int4(ul, vl, RW, RH) : make_int4(0, 0, 0, 0);
     __builtin_amdgcn_wave_barrier();
+    }  // passes
   }
 }
 
@@ -1424,6 +1461,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // Q row of the wave: W zero entries before and after it
   T *qrow = region + lds_cap + wave * (pmax + 2 * W) + W;
+  const T uscl = T(P.u_scl);
   if (lane < W) qrow[lane - W] = T(0);
   __shared__ int red[4 * kWaves];
   TileMap tm;
@@ -1554,7 +1592,7 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
                 const int k = kf + kKT * q + lane;
                 const bool kin = k < P.nz;
                 Sym<T> sv;
-                sym_voxel(P, a12s, c, double(kin ? k : P.nz - 1), lo, sv);
+                sym_voxel(uscl, a12s, c, double(kin ? k : P.nz - 1), lo, sv);
                 const T gq = sym_gather_n<W>(J, sv, qrow + (sv.p - J - 1));
                 contrib[q] = kin ? gq * sv.rho : T(0);
               }
