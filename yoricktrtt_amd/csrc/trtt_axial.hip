@@ -708,6 +708,7 @@ __device__ __forceinline__ void sym_profile_n(int J, const Sym<T> &s, T *pp, T v
   }
 }
 
+
 template <int W, typename T>
 __device__ __forceinline__ T sym_gather_n(int J, const Sym<T> &s, const T *qq) {
   switch (J) {
@@ -1118,8 +1119,9 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     // that window does not fit the LDS capacity (atomic flush only: a
     // scratch window has one slot per view and tile), in two passes over
     // two halves of the tile (its rows of columns split, else its columns
-    // of columns), each with its own window and flush.  A view whose halves
-    // do not fit either takes the direct path.
+    // of columns), else in four passes over its quadrants, each pass with
+    // its own window and flush.  A view whose quadrants do not fit either
+    // takes the direct path.
     const int cidx = lane - loff;
     const bool half_rows = cidx / CX < CY / 2, half_cols = cidx % CX < CX / 2;
     int split = 0;
@@ -1134,10 +1136,19 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
       else if (window_fits(cl, inview && half_cols, wcap, a, b, c, d) &&
                window_fits(cl, inview && !half_cols, wcap, e, f, g, h))
         split = 2;
+      else if (window_fits(cl, inview && half_rows && half_cols, wcap, a, b, c, d) &&
+               window_fits(cl, inview && !half_rows && half_cols, wcap, e, f, g, h) &&
+               window_fits(cl, inview && half_rows && !half_cols, wcap, a, b, c, d) &&
+               window_fits(cl, inview && !half_rows && !half_cols, wcap, e, f, g, h))
+        split = 3;
     }
-    for (int pass = 0; pass < (split ? 2 : 1); ++pass) {
-    const bool live =
-        inview && (split == 0 || (pass == 0) == (split == 1 ? half_rows : half_cols));
+    const int npass = split == 0 ? 1 : (split == 3 ? 4 : 2);
+    for (int pass = 0; pass < npass; ++pass) {
+    // pass p of the quadrants: rows half p & 1, columns half p >> 1
+    const bool in_rows = (pass & 1) == 0 ? half_rows : !half_rows;
+    const bool in_cols = (split == 3 ? pass >> 1 : pass) == 0 ? half_cols : !half_cols;
+    const bool live = inview && (split == 0 || (split == 1 && in_rows) ||
+                                 (split == 2 && in_cols) || (split == 3 && in_rows && in_cols));
     int ul = ul0, ur = ur0, vl = vl0, vr = vr0;
     bool in_lds = fits0;
     if (split) in_lds = window_fits(cl, live, wcap, ul, ur, vl, vr);
@@ -1174,6 +1185,14 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
         xb = two ? vol1[idx] : T(0);
       };
       if (pend) load_col(__builtin_ctzll(pend));
+      // The column loop.  JJ >= 0 (closed-form footprint, plain profile
+      // steps): every column of the view takes the window width 2 JJ + 3,
+      // JJ the largest J of its columns (a narrower column gets extra bounds
+      // of weight 0 on each side), so the loop dispatches on J and on the
+      // pass mode once per view instead of once per column.  JJ < 0: the
+      // dispatch per column.
+      auto columns = [&](auto jc) {
+      constexpr int JJ = decltype(jc)::value;
       while (pend) {
         const int cc = __builtin_ctzll(pend);
         pend &= pend - 1;
@@ -1187,6 +1206,11 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
           Col<T> c;
           int lo, span, J;
           bcast_sym(cl, cc, c, lo, span, J);
+          if constexpr (JJ >= 0) {
+            lo -= JJ - J;
+            span += 2 * (JJ - J);
+            J = JJ;
+          }
           const int npv = c.vr - c.vl;
           const bool use_prof = in_lds && span <= pmax && npv <= W && 2 * J + 3 <= W;
           T cv[W];
@@ -1201,8 +1225,15 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
             // lanes past the last slice share the clamped slice's window:
             // they add their zeros into the W margin entries in front of the
             // profile (never read), so no two lanes of a plain step meet
-            sym_profile_n<W>(J, sv, kin ? prof + (sv.p - J - 1) : prof - W,
-                             kin ? x * sv.rho : T(0), plain);
+            T *const pp = kin ? prof + (sv.p - J - 1) : prof - W;
+            const T val = kin ? x * sv.rho : T(0);
+            if constexpr (JJ >= 0) {
+              T w[2 * JJ + 3];
+              sym_weights<JJ>(sv, w);
+              accumulate_steps<2 * JJ + 3>(w, pp, val, 1);
+            } else {
+              sym_profile_n<W>(J, sv, pp, val, plain);
+            }
             __builtin_amdgcn_wave_barrier();
             // outer product P x cv over the column's pixels in the window
             // [ul, ur) (which lies on the detector): pixel a + t is profile
@@ -1294,6 +1325,19 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
             slow_voxel<true, TAB, T>(P, c, v, val, out, nullptr, dummy);
           }
         }
+      }
+      };
+      if constexpr (TAB == 0) {
+        // J of the view: the largest over its live columns (wave-uniform)
+        const int jv = bcast_i(wred_max(live ? int(unsigned(cl.span_j) >> 16) : 0), 0);
+        if (plain == 1 && jv == 3 && 9 <= W)
+          columns(std::integral_constant<int, 3>{});
+        else if (plain == 1 && jv == 2)
+          columns(std::integral_constant<int, 2>{});
+        else
+          columns(std::integral_constant<int, -1>{});
+      } else {
+        columns(std::integral_constant<int, -1>{});
       }
     }
     if (in_lds && flush) {
@@ -1541,6 +1585,9 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       __syncthreads();
       if (any) {
         const unsigned long long live_mask = __ballot(live) >> loff;
+        // (J is dispatched per column here: a dispatch once per view, with
+        // the loop instantiated per J as in the forward, spilled more
+        // registers and ran slower, 22.8 -> 26.0 ms on the 256^3 cone.)
 #pragma unroll 1
         for (int cc = 0; cc < NCOL; ++cc) {
           if (!((live_mask >> cc) & 1ull)) continue;
@@ -1587,6 +1634,8 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
                 for (int u = lane; u < cnt; u += 64) qa[u] = q_n<W>(npv, src0 + u, pitch, cv);
               }
               __builtin_amdgcn_wave_barrier();
+              // (one dispatch on J for the VPL voxels together, which lets
+              // their LDS reads overlap, spilled more registers: slower)
 #pragma unroll
               for (int q = 0; q < VPL; ++q) {
                 const int k = kf + kKT * q + lane;
