@@ -1036,7 +1036,10 @@ __device__ __forceinline__ bool window_fits(const ColLane<T> &cl, bool m, int ca
 #define TRTT_BWD_OCC
 #endif
 
-template <bool CONE, int W, int CX, int CY, int TAB, typename T>
+// TWO: some view interpolates between two phases; without, every view
+// reads one phase (t0 = t1) with the weight w0 + w1, and the second phase's
+// pointer, weight and branches leave the column loop.
+template <bool CONE, int W, int CX, int CY, int TAB, bool TWO, typename T>
 __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__restrict__ vol,
                                           T *__restrict__ sino, int views_per_block, int wcap,
                                           int pmax, int view_base, int view_end,
@@ -1103,10 +1106,10 @@ __global__ __launch_bounds__(64) TRTT_FWD_OCC void fwd(DevPlan P, const T *__res
     // stores would otherwise force a reload per column)
     const double a12 = g.a12;
     const T a12s = T(g.a12 * P.u_scl);
-    const T w0 = T(g.w0), w1 = T(g.w1);
-    const bool two = g.w1 != 0.0;
+    const T w0 = TWO ? T(g.w0) : T(g.w0 + g.w1), w1 = T(g.w1);
+    const bool two = TWO && g.w1 != 0.0;
     const T *vol0 = vol + long(g.t0) * P.vox_per_phase;
-    const T *vol1 = vol + long(g.t1) * P.vox_per_phase;
+    const T *vol1 = TWO ? vol + long(g.t1) * P.vox_per_phase : vol0;
 
     if (!multi) {
       column_table<CONE, TAB, CX, CY, NCOL>(P, g, tm, 0, lane, kf, kl, cl);
@@ -1813,16 +1816,17 @@ size_t axial_scratch_bytes(const DevPlan &P, int chunk, const AxialTuning &tu) {
 template <bool CONE, int W, int CX, int CY, typename T>
 void launch_fwd(const DevPlan &P, dim3 grid, int lds_bytes, hipStream_t st, const T *volk, T *out,
                 int vpb, int wcap, int pmax, int vb, int ve, T *scratch, int4 *hdr, int plain,
-                int flush, int ntiles, int xcds) {
-#define TRTT_FWD(TB)                                                                           \
-  hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, TB, T>), grid, dim3(64), lds_bytes, st, P, volk, \
-                     out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush, ntiles, xcds)
-  if (!P.table)
-    TRTT_FWD(0);
-  else if (P.table_deg == 2)
-    TRTT_FWD(2);
-  else
-    TRTT_FWD(3);
+                int flush, int ntiles, int xcds, bool two) {
+#define TRTT_FWD(TB, TW)                                                                   \
+  hipLaunchKernelGGL((axial::fwd<CONE, W, CX, CY, TB, TW, T>), grid, dim3(64), lds_bytes, st, P, \
+                     volk, out, vpb, wcap, pmax, vb, ve, scratch, hdr, plain, flush, ntiles, xcds)
+  if (!P.table) {
+    if (two) TRTT_FWD(0, true); else TRTT_FWD(0, false);
+  } else if (P.table_deg == 2) {
+    if (two) TRTT_FWD(2, true); else TRTT_FWD(2, false);
+  } else {
+    if (two) TRTT_FWD(3, true); else TRTT_FWD(3, false);
+  }
 #undef TRTT_FWD
 }
 
@@ -1889,15 +1893,15 @@ hipError_t launch_axial_t(const DevPlan &P, int job, const void *in, void *out,
       switch (tu.fwd_tile) {
         case 1:
           launch_fwd<CONE, W, 4, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds);
+                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds, two != 0);
           break;
         case 2:
           launch_fwd<CONE, W, 8, 8, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds);
+                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds, two != 0);
           break;
         default:
           launch_fwd<CONE, W, 8, 4, T>(P, grid, lds_bytes, st, volk, o, views_per_block, wcap, pmax,
-                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds);
+                                       vb, ve, sc, hd, plain, flush, int(ntiles), xcds, two != 0);
       }
       if (ev) hipEventRecord(ev[2 * npair + 1], st);
       ++npair;
