@@ -33,6 +33,12 @@ NVIEWS = 360
 RSC, RSD = 1000.0, 1500.0
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP32_PEAK = 157.3e12       # FLOP/s, MI355X FP32 vector
+# Footprint weights of one pass over the 360 views (sum over voxels and views
+# of npix_u * npix_v, the reference's window sizes), counted with the oracle's
+# FP64 geometry pass by tests/golden/make_footprint_count.py and frozen in
+# tests/golden/footprint_count_cone256.json; each weight is one multiply-add
+# in the forward and one in the back projection: 4 flops per weight a pair.
+FOOTPRINT_WEIGHTS = 290543347804
 
 
 def geometry():
@@ -631,6 +637,8 @@ def main():
         "pair_fracs": {
             "hbm_per_view_bytes": round((fwd_b + bwd_b) * pairs_per_s / HBM_PEAK, 4),
             "hbm_compulsory_bytes": round(comp_b * pairs_per_s / HBM_PEAK, 6),
+            # 4 flops per footprint weight (both models use the same windows)
+            "fp32_footprint_flops": round(4 * FOOTPRINT_WEIGHTS * pairs_per_s / FP32_PEAK, 4),
         },
     }
     result["roofline"]["traffic"] = measured_traffic(tuning, result["roofline"]["kernel"])

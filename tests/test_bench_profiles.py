@@ -46,3 +46,22 @@ def test_forward_traffic_within_the_per_view_bytes():
     # and one projection write per view
     assert vals["fwd_launches"] == 1
     assert 0 < vals["fwd_traffic_bytes_per_launch"] <= vals["fwd_algorithmic_bytes_per_launch"]
+
+
+def test_footprint_weights_match_the_fixture():
+    # bench.py's flop count is the frozen oracle count of the same workload
+    doc = json.load(open(os.path.join(HERE, "golden", "footprint_count_cone256.json")))
+    assert bench.FOOTPRINT_WEIGHTS == doc["total"]
+    assert (bench.NX, bench.NY, bench.NZ, bench.NU, bench.NV, bench.NVIEWS) == (256, 256, 256, 384,
+                                                                                 512, 360)
+    assert (bench.RSC, bench.RSD) == (1000.0, 1500.0)
+    import importlib.util
+
+    import numpy as np
+
+    spec = importlib.util.spec_from_file_location(
+        "make_footprint_count", os.path.join(HERE, "golden", "make_footprint_count.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    a, b = bench.geometry(), mk.bench_geometry()
+    assert np.array_equal(a.xos, b.xos) and np.array_equal(a.xsd, b.xsd)

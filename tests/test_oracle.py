@@ -134,6 +134,57 @@ def test_single_voxel_mass(oracle):
 
 
 @pytest.mark.parametrize("cone", [True, False])
+def test_count_pairs_brute_force(oracle, cone):
+    """The footprint-weight count equals the number of nonzero entries of the
+    projection matrix, found by projecting every unit voxel (a detector
+    small enough that edge voxels are clipped).  The parallel views take
+    detector offsets of a fraction of a pixel: on the unshifted grid some
+    footprints end exactly on a pixel edge, and the window then holds a
+    pixel of weight 0 that the loop visits but the matrix does not store."""
+    from yoricktrtt_amd.geometry import parallel_geometry_3d
+
+    nx, ny, nz, nu, nv = 6, 5, 4, 11, 9
+    for theta in (0.37, 2.1):
+        if cone:
+            xos, xsd = _cone_view(theta, nx, ny, nz, nu, nv, Rsc=20.0, Rsd=35.0)
+        else:
+            g = parallel_geometry_3d(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, [theta], u_off=[0.31],
+                                     v_off=[0.17])
+            xos, xsd = g.xos[0], g.xsd[0]
+        nnz = 0
+        for q in range(nx * ny * nz):
+            vol = np.zeros(nx * ny * nz)
+            vol[q] = 1.0
+            out = oracle.sb3d(np.zeros((nv, nu)), vol.reshape(nz, ny, nx), nx, ny, nz, 1.0, nu, nv,
+                              1.0, 1.0, xos, xsd, cone, 0)
+            nnz += int(np.count_nonzero(out))
+        assert oracle.count_pairs(nx, ny, nz, 1.0, nu, nv, 1.0, 1.0, xos, xsd, cone) == nnz
+
+
+def test_footprint_count_fixture(oracle):
+    """The frozen weight count of the cone 256^3 benchmark: sampled views
+    recounted with the oracle's geometry pass, and the totals consistent."""
+    import json
+    import importlib.util
+
+    with open(os.path.join(GOLDEN, "footprint_count_cone256.json")) as f:
+        doc = json.load(f)
+    assert len(doc["per_view"]) == 360
+    assert doc["total"] == sum(doc["per_view"])
+    assert doc["flops_per_pair"] == 4 * doc["total"]
+    spec = importlib.util.spec_from_file_location(
+        "make_footprint_count", os.path.join(GOLDEN, "make_footprint_count.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    g = mk.bench_geometry()
+    views = [0, 37, 90, 181, 359]
+    assert mk.count_views(oracle, g, views) == [doc["per_view"][v] for v in views]
+    # every voxel projects inside the detector, so a weight count near the
+    # 7 x 7 of a magnification-1.5 cubic footprint is expected
+    assert 40.0 < doc["mean_per_voxel_view"] < 60.0
+
+
+@pytest.mark.parametrize("cone", [True, False])
 @pytest.mark.parametrize("table", [False, True])
 def test_oracle_adjoint_and_linearity(oracle, cone, table):
     rng = np.random.default_rng(7)
