@@ -970,16 +970,22 @@ __device__ __forceinline__ void bcast_sym(const ColLane<T> &src, int cc, Col<T> 
   J = int(unsigned(sj) >> 16);
 }
 
-__device__ __forceinline__ int wred_min(int x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
-  return x;
-}
-__device__ __forceinline__ int wred_max(int x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
-  return x;
-}
+}  // namespace axial
+}  // namespace trtt
+
+// Wave reductions of the ROCm device library (DPP lane shifts and a
+// readlane, no LDS round trips), over the active lanes: every caller runs
+// them with the whole wave active.
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+
+namespace trtt {
+namespace axial {
+
+// (the earlier butterfly of __shfl_xor steps, six ds_bpermute round trips
+// per reduction, cost the 256^3 cone forward 0.8 ms and the backward 0.8 ms)
+__device__ __forceinline__ int wred_min(int x) { return __ockl_wfred_min_i32(x); }
+__device__ __forceinline__ int wred_max(int x) { return __ockl_wfred_max_i32(x); }
 
 // Block window: union over the live columns of the four waves.
 template <typename T>
