@@ -1569,10 +1569,19 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       const bool fixed = in_lds && RW <= kPitch && kPitch * RH <= lds_cap;
       const int pitch = fixed ? kPitch : RW;
       if (fixed) {
+        // eight rows' loads in flight at a time (one load and its wait per
+        // row left the view's staging latency exposed)
         const int cI = int(threadIdx.x);
         if (cI < RW) {
           long o = long(vl) * P.nu + (ul + cI);
-          for (int r = 0; r < RH; ++r, o += P.nu) region[r * kPitch + cI] = in[o];
+          for (int r = 0; r < RH; r += 8, o += 8 * long(P.nu)) {
+            T t[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t[q] = r + q < RH ? in[o + q * long(P.nu)] : T(0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              if (r + q < RH) region[(r + q) * kPitch + cI] = t[q];
+          }
         }
       } else if (in_lds) {
         // flat over the window with the source offset advanced
