@@ -1568,7 +1568,20 @@ __global__ __launch_bounds__(kThreads) TRTT_BWD_OCC void bwd(DevPlan P, const T 
       // stages column c of every row.
       const bool fixed = in_lds && RW <= kPitch && kPitch * RH <= lds_cap;
       const int pitch = fixed ? kPitch : RW;
-      if (fixed) {
+      if (fixed && sizeof(T) == 4) {
+        // loads straight into LDS (no registers, no LDS write instructions,
+        // every row in flight at once): lane L of wave w writes column
+        // 64 w + L of the row, the wave's destination being uniform
+        const int cI = int(threadIdx.x);
+        if (cI < RW) {
+          const T *src = in + (long(vl) * P.nu + (ul + cI));
+          T *dst = region + (cI & ~63);
+          for (int r = 0; r < RH; ++r, src += P.nu, dst += kPitch)
+            __builtin_amdgcn_global_load_lds(src, dst, 4, 0, 0);
+        }
+        // the loads' LDS writes land before this wave reaches the barrier
+        __builtin_amdgcn_s_waitcnt(0);
+      } else if (fixed) {
         // eight rows' loads in flight at a time (one load and its wait per
         // row left the view's staging latency exposed)
         const int cI = int(threadIdx.x);
