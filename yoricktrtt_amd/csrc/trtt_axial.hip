@@ -1023,10 +1023,11 @@ __device__ __forceinline__ bool window_fits(const ColLane<T> &cl, bool m, int ca
 // ---------------------------------------------------------------------------
 // Occupancy targets (waves per SIMD) for the register allocator, 0 leaves
 // the allocation to the compiler.  Measured on the 256^3 cone benchmark:
-// forward 5 (96 VGPRs) beats 4 (106) and 6 (80, spills); backward 6 (78)
-// beats 4 (127) and 5 (90).
+// forward 6 (80 VGPRs, a few spilled; its LDS then allows 21 waves per CU)
+// beats 5 (96 VGPRs, 20 waves: 28.2 against 27.9 ms) and 4 (128 VGPRs:
+// 32 ms); backward 6 (78) beats 4 (127) and 5 (90).
 #ifndef TRTT_AXIAL_FWD_WAVES
-#define TRTT_AXIAL_FWD_WAVES 5
+#define TRTT_AXIAL_FWD_WAVES 6
 #endif
 #ifndef TRTT_AXIAL_BWD_WAVES
 #define TRTT_AXIAL_BWD_WAVES 6
