@@ -106,7 +106,8 @@ __device__ __forceinline__ void clampw(double c, double hs, int n, int &lo, int 
 template <typename T>
 struct Col {
   double vc;       // transaxial centre
-  T vcf;           // vc - vl (single precision is exact enough: |vc - vl| < a few pixels)
+  T xvs;           // -(vc - vl) sv: the v CDF abscissa of the window start (|vc - vl| is a
+                   // few pixels, so single precision is exact enough)
   double uc0, duc; // uc(k) = uc0 + duc k
   T rho;           // rho0 / du
   T hu0;           // hu / du
@@ -163,7 +164,7 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.ca = T(kC24 / (P.hu_init * magn));
     c.inv_a = T(1.0 / a);
     clampw(c.vc, P.hs0 * hv, P.nv, c.vl, c.vr);
-    c.vcf = T(c.vc - double(c.vl));
+    c.xvs = -T(c.vc - double(c.vl)) * c.sv;
     if (TAB) {
       // trtt_3D_projectors.c:267-275: scl_norm_v / step and the start
       const double sdv = (1.0 / hv) / P.step;
@@ -183,7 +184,7 @@ __device__ __forceinline__ void column(const DevPlan &P, const ViewConst &g, int
     c.ca = T(kC24 / g.pb_hu);
     c.inv_a = T(0);
     clampw(c.vc, g.pb_hsv, P.nv, c.vl, c.vr);
-    c.vcf = T(c.vc - double(c.vl));
+    c.xvs = -T(c.vc - double(c.vl)) * c.sv;
     if (TAB) {
       const double sdv = g.pb_sv / P.step;
       c.sdv = sdv;
@@ -420,17 +421,19 @@ __device__ __forceinline__ float from_next_lane(float v) {
 }
 
 // v weights of the column: lane m evaluates the CDF at the m-th v pixel
-// bound, one shifted difference gives the weight of pixel m in lane m (zero
-// from the window's end on), and readlane broadcasts the W weights.  (An
+// bound, one shifted difference gives the weight of pixel m in lane m, and
+// readlane broadcasts the W weights.  (An
 // LDS slot per wave read back with broadcast reads instead was slower in
 // both kernels: the backward's extra LDS costs a workgroup per CU.)
 template <int W, int TAB = 0, typename T>
 __device__ __forceinline__ void vweights(const DevPlan &P, const Col<T> &c, int lane, T (&cv)[W]) {
   const T f = TAB ? table_cdf<TAB, T>(P, fma(double(lane), c.sdv, c.xv0))
-                  : cdf3c<T>((T(lane) - c.vcf) * c.sv);
-  // the shift runs on all lanes (a lane reading a disabled lane gets 0)
+                  : cdf3c<T>(c.xvs + T(lane) * c.sv);
+  // the shift runs on all lanes (a lane reading a disabled lane gets 0);
+  // the weights of rows m >= npv are not zeroed: every row loop that
+  // takes them (outer products, Q rows) stops at the column's npv rows
   const T fn = from_next_lane(f);
-  const T d = lane < c.vr - c.vl ? fn - f : T(0);
+  const T d = fn - f;
 #pragma unroll
   for (int m = 0; m < W; ++m) cv[m] = bcast(d, m);
 }
@@ -764,7 +767,7 @@ __device__ void slow_voxel(const DevPlan &P, const Col<T> &c, const Vox<T> &v, T
   const int npu = v.ur - ul, npv = c.vr - c.vl;
   if (npu <= 0 || npv <= 0) return;
   const T xu0 = T(double(ul) - v.uc) * v.su;
-  const T xv0 = -c.vcf * c.sv;
+  const T xv0 = c.xvs;
   // tabulated footprint: u coordinates from the window start ul (>= v.ul)
   auto fu = [&](int l) {
     return TAB ? table_cdf<TAB, T>(P, fma(double(ul - v.ul + l), v.sdu, v.xu0))
@@ -928,7 +931,7 @@ __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> 
     c.hu0d = bcast(src.c.hu0d, cc);
     c.inv_ad = bcast(src.c.inv_ad, cc);
   }
-  c.vcf = bcast(src.c.vcf, cc);
+  c.xvs = bcast(src.c.xvs, cc);
   c.uc0 = bcast(src.c.uc0, cc);
   c.duc = bcast(src.c.duc, cc);
   c.rho = bcast(src.c.rho, cc);
@@ -953,7 +956,7 @@ __device__ __forceinline__ void bcast_col(const ColLane<T> &src, int cc, Col<T> 
 template <typename T>
 __device__ __forceinline__ void bcast_sym(const ColLane<T> &src, int cc, Col<T> &c, int &lo,
                                           int &span, int &J) {
-  c.vcf = bcast(src.c.vcf, cc);
+  c.xvs = bcast(src.c.xvs, cc);
   c.uc0 = bcast(src.c.uc0, cc);
   c.duc = bcast(src.c.duc, cc);
   c.rho = bcast(src.c.rho, cc);
